@@ -1,0 +1,137 @@
+/*
+ * qrdm_amd.h -- C ABI of the MI355X DM-QR library (libqrdm_amd.so).
+ *
+ * QR factorisation with Deviation Maximization column pivoting, A*P = Q*R,
+ * computed on one AMD Instinct MI355X (gfx950) by hand-written HIP kernels.
+ * All signatures use plain C types; lapack_int is 32-bit int.
+ */
+#ifndef QRDM_AMD_H_
+#define QRDM_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Drop-in for the reference's public entry point
+ *   lapack_int dgeqrdm(int matrix_layout, lapack_int m, lapack_int n,
+ *                      double* a, lapack_int lda, lapack_int* jpvt,
+ *                      double* tau, lapack_int* ncols, double* thres,
+ *                      int DM_procedure);
+ * (include/QRDM.h:35-38, src/dgeqrdm.c:5-16, src/dgeqrdm_work.c:420-485).
+ *
+ * All arrays are HOST memory owned by the caller and updated in place:
+ *   a      lda x n column-major; on exit R (upper trapezoid) and the
+ *          Householder vectors below the diagonal;
+ *   jpvt   on entry 0 for free columns (non-zero = fixed leading column);
+ *          on exit 1-based: column j of A*P is column jpvt[j] of A;
+ *   tau    min(m,n) reflector scalars;
+ *   ncols  on entry ncols[0] in {1,2,3} enables the stopping test
+ *          (eta = eps*n, eps*sqrt(n), thres[2]); on exit ncols[it] is the
+ *          number of columns reduced at DM step it (room for n entries);
+ *   thres  thres[0] = delta in [0,1] (cosine bound), thres[1] = tau in [0,1]
+ *          (norm threshold), thres[2] = eta for stop mode 3;
+ *   nb     maximum DM block (k_max), 1..64 on this device.
+ * Returns 0 on success; -1 for an illegal argument (a message is printed, as
+ * the reference's xerbla_ does); -10 for nb > 64, which this device build
+ * does not support (the reference accepts any nb > 0); -13 when the input
+ * holds a NaN or an infinity (A is then left untouched);
+ * <= -1000 for a device error (see qrdm_last_error).  matrix_layout 101 is
+ * accepted and treated as column-major, as the reference's DM code does.
+ */
+int dgeqrdm(int matrix_layout, int m, int n, double *a, int lda, int *jpvt, double *tau,
+            int *ncols, double *thres, int nb);
+
+/*
+ * Device-resident variant of dgeqrdm: d_a (lda x n) and d_tau (min(m,n)) are
+ * DEVICE pointers on the current HIP device; jpvt, ncols, thres stay in host
+ * memory.  Work is queued on `stream` (a hipStream_t, NULL = default stream)
+ * and the call returns after it completes.  No counterpart in the reference:
+ * it is the form the host entry point above runs after its copies, exposed
+ * for callers whose matrix already lives in HBM.
+ */
+int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
+                     double *d_tau, int *ncols, const double *thres, int nb, void *stream);
+
+/* Per-step record of the DM loop (debugging and step-by-step parity). */
+typedef struct {
+  int j;         /* 0-based first column of the step */
+  int kmax;      /* min(nb, m - j, n - j) */
+  int nc;        /* number of norm candidates (> tau * max, at most kmax) */
+  int fjb;       /* columns accepted by the cosine test */
+  int fjb_cmp;   /* columns reduced by the panel (<= fjb) */
+  int nswap;     /* column swaps made by the permutation */
+  double maxnrm; /* largest partial norm after the step's downdate */
+  int sel[64];   /* accepted columns relative to j, in acceptance order */
+} qrdm_step;
+
+/*
+ * qrdm_dgeqrdm_dev plus a trace: up to ntrace records are written to trace
+ * and the number of steps taken to *nsteps.  Same arithmetic as above.
+ */
+int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
+                           double *d_tau, int *ncols, const double *thres, int nb, void *stream,
+                           qrdm_step *trace, int ntrace, int *nsteps);
+
+/*
+ * Unpivoted Householder QR on the device (LAPACK dgeqrf semantics; the
+ * reference's QRDM.QRF forwards to LAPACKE_dgeqrf, QRDM_wrapper.c:44-68).
+ * d_a, d_tau device pointers.
+ */
+int qrdm_dgeqrf_dev(int m, int n, double *d_a, int lda, double *d_tau, void *stream);
+
+/*
+ * C := Q C (trans = 0) or Q^T C (trans = 1) from the left, Q = H(1)...H(k)
+ * held in d_a/d_tau as dgeqrdm or dgeqrf leave them (the reference's
+ * QRDM.DORMQR forwards to LAPACKE_dormqr(layout,'L','N',...),
+ * QRDM_wrapper.c:104-126).  All matrices are device pointers.
+ */
+int qrdm_dormqr_dev(int trans, int m, int n, int k, const double *d_a, int lda,
+                    const double *d_tau, double *d_c, int ldc, void *stream);
+
+/* Host-pointer forms of the two routines above (copies in and out). */
+int qrdm_dgeqrf(int m, int n, double *a, int lda, double *tau);
+int qrdm_dormqr(int trans, int m, int n, int k, const double *a, int lda, const double *tau,
+                double *c, int ldc);
+
+/*
+ * Host index walk of one DM permutation (permute_marked,
+ * src/dgeqrdm_work.c:149-262, with marks 0/1): given the accepted columns
+ * sel[0..nsel) (relative positions < ncols, acceptance order), writes the
+ * swap sequence it performs as pairs swaps[2*i], swaps[2*i+1] and returns
+ * their number (at most ncols: every swap advances the front pointer).  No counterpart in the reference's ABI: it
+ * is exposed so the host side of the permutation can be checked without a
+ * GPU.  Returns -1 for invalid input.
+ */
+int qrdm_permute_walk(int ncols, const int *sel, int nsel, int *swaps);
+
+/*
+ * Kernel-time accounting for measurement.  While enabled, the driver records
+ * HIP events (on the stream it launches on) around each class of kernel and
+ * accumulates, per class, the summed event time in ms, the launch count and
+ * the algorithmic work of those launches.  Classes:
+ *   0 trailing update (larfb: V^T C, T^T W, C - V W)   work = flops
+ *   1 norm downdate/recompute                          work = HBM bytes
+ *   2 DM selection (top-k, candidate Gram, greedy)     work = HBM bytes
+ *   3 panel (Householder + V + T)                      work = flops
+ *   4 column permutation                               work = HBM bytes
+ *   5 initial norms                                    work = HBM bytes
+ * qrdm_profile_read fills out[3*c + {0,1,2}] = {ms, launches, work} for
+ * c < nclass and returns the number of classes.  No counterpart in the
+ * reference (it has no timing facility).
+ */
+void qrdm_profile_enable(int on);
+void qrdm_profile_reset(void);
+int qrdm_profile_read(double *out, int nclass);
+
+/* Text of the last device error seen by this thread's calls ("" if none). */
+const char *qrdm_last_error(void);
+
+/* Library version string. */
+const char *qrdm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QRDM_AMD_H_ */

@@ -1,0 +1,118 @@
+"""qrdm_amd -- QR with Deviation Maximization pivoting on AMD Instinct MI355X.
+
+The computation lives in libqrdm_amd.so (HIP kernels for gfx950 and a C++
+host driver behind a C ABI, include/qrdm_amd.h).  This package loads it and
+offers thin helpers for callers holding device tensors.  The Python drop-in
+for the reference's module is the extension "QRDM" built next to this
+package (import QRDM).  Loading fails loudly if the library is missing: there
+is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqrdm_amd.so")
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Step(ctypes.Structure):
+    """qrdm_step of include/qrdm_amd.h."""
+    _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
+                ("fjb", ctypes.c_int), ("fjb_cmp", ctypes.c_int), ("nswap", ctypes.c_int),
+                ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64)]
+
+
+# Every entry point include/qrdm_amd.h declares.
+EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrf_dev",
+           "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
+           "qrdm_profile_enable", "qrdm_profile_reset", "qrdm_profile_read",
+           "qrdm_last_error", "qrdm_version")
+
+PROFILE_CLASSES = ("larfb", "norm_update", "select", "panel", "permute", "init_norms")
+
+
+def lib():
+    """The loaded libqrdm_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqrdm_amd.so is not built; run python -m qrdm_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        V = ctypes.c_void_p
+        L.dgeqrdm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                              _dp, _ip, _dp, ctypes.c_int]
+        L.qrdm_dgeqrdm_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, ctypes.c_int,
+                                       _ip, V, _ip, _dp, ctypes.c_int, V]
+        L.qrdm_dgeqrdm_dev_trace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
+                                             ctypes.c_int, _ip, V, _ip, _dp, ctypes.c_int, V,
+                                             ctypes.POINTER(Step), ctypes.c_int, _ip]
+        L.qrdm_dgeqrf_dev.argtypes = [ctypes.c_int, ctypes.c_int, V, ctypes.c_int, V, V]
+        L.qrdm_dormqr_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
+                                      ctypes.c_int, V, V, ctypes.c_int, V]
+        L.qrdm_dgeqrf.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]
+        L.qrdm_dormqr.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                  ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.qrdm_permute_walk.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip]
+        L.qrdm_profile_enable.argtypes = [ctypes.c_int]
+        L.qrdm_profile_read.argtypes = [_dp, ctypes.c_int]
+        L.qrdm_profile_read.restype = ctypes.c_int
+        L.qrdm_last_error.restype = ctypes.c_char_p
+        L.qrdm_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().qrdm_last_error().decode()
+
+
+def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
+                layout=102):
+    """DM-QR in place of a device matrix.
+
+    a: contiguous float64 device tensor of shape (n, lda) -- its memory is the
+    column-major m x n matrix with leading dimension lda >= m; tau: float64
+    device tensor with min(m, n) entries; jpvt, ncols: int32 numpy arrays;
+    thres: float64 numpy array [delta, tau(, eta)].  stream: hipStream_t as
+    an int (torch.cuda.current_stream().cuda_stream), None = default stream.
+    Returns (info, steps), steps = per-step records when trace is set.
+    """
+    import numpy as np
+    L = lib()
+    if a.dim() != 2 or not a.is_contiguous() or a.element_size() != 8:
+        raise ValueError("a must be a contiguous float64 (n, lda) tensor")
+    n, lda = int(a.shape[0]), int(a.shape[1])
+    if not (1 <= m <= lda):
+        raise ValueError("need 1 <= m <= lda")
+    s = ctypes.c_void_p(stream if stream is not None else 0)
+    thres = np.ascontiguousarray(thres, dtype=np.float64)
+    args = (layout, m, n, ctypes.c_void_p(a.data_ptr()), lda, jpvt.ctypes.data_as(_ip),
+            ctypes.c_void_p(tau.data_ptr()), ncols.ctypes.data_as(_ip),
+            thres.ctypes.data_as(_dp), nb, s)
+    if trace:
+        cap = max(1, min(m, n))
+        tr = (Step * cap)()
+        ns = ctypes.c_int(0)
+        info = L.qrdm_dgeqrdm_dev_trace(*args, tr, cap, ctypes.byref(ns))
+        steps = [dict(j=t.j, kmax=t.kmax, nc=t.nc, fjb=t.fjb, fjb_cmp=t.fjb_cmp, nswap=t.nswap,
+                      maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]))
+                 for t in tr[:ns.value]]
+        return info, steps
+    return L.qrdm_dgeqrdm_dev(*args), None
+
+
+def profile(enable=None, reset=False):
+    """Reads (and optionally resets / toggles) the library's kernel-time
+    accounting: {class: (ms, launches, work)}."""
+    import numpy as np
+    L = lib()
+    if enable is not None:
+        L.qrdm_profile_enable(1 if enable else 0)
+    out = np.zeros(3 * len(PROFILE_CLASSES))
+    L.qrdm_profile_read(out.ctypes.data_as(_dp), len(PROFILE_CLASSES))
+    if reset:
+        L.qrdm_profile_reset()
+    return {c: tuple(out[3 * i:3 * i + 3]) for i, c in enumerate(PROFILE_CLASSES)}

@@ -1,0 +1,664 @@
+// driver.hip -- host driver of the DM-QR library and its C ABI.
+//
+// dgeqrdm_work's control flow (src/dgeqrdm_work.c:420-838) on the host, the
+// arithmetic in the kernels of kernels.hip.  Per DM step: top-k of the partial
+// norms -> candidate Gram -> greedy acceptance -> (host) permute_marked index
+// walk -> column permutation -> Householder panel -> T -> trailing update ->
+// norm downdate.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qrdm_amd.h"
+#include "qrdm_internal.h"
+
+using namespace qrdm;
+
+namespace {
+
+thread_local std::string g_err;
+
+constexpr double kEps = 1.1102230246251565e-16;  // LAPACK dlamch('E') = 2^-53
+
+struct HipError {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIPCHK(x)                                      \
+  do {                                                 \
+    hipError_t e_ = (x);                               \
+    if (e_ != hipSuccess) throw HipError{e_, #x};      \
+  } while (0)
+
+// Grow-only device workspace, one per device, reused across calls.
+struct Workspace {
+  size_t bytes = 0;
+  char* base = nullptr;
+  void* pinned = nullptr;  // host staging for the per-step readback
+  size_t pinned_bytes = 0;
+};
+
+std::mutex g_mu;
+Workspace g_ws[64];
+
+// Kernel-time accounting (qrdm_profile_*): HIP events around each class of
+// kernels on the launch stream, resolved after the call's final sync.
+constexpr int kNumClasses = 6;
+enum { PC_LARFB = 0, PC_NORM = 1, PC_SELECT = 2, PC_PANEL = 3, PC_PERM = 4, PC_INIT = 5 };
+struct Profiler {
+  bool on = false;
+  double ms[kNumClasses] = {}, launches[kNumClasses] = {}, work[kNumClasses] = {};
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+    double work;
+  };
+  std::vector<Rec> open;
+  hipEvent_t ev() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  // begin/end bracket one region; work is its algorithmic flops or bytes
+  int begin(int cls, hipStream_t s) {
+    if (!on) return -1;
+    Rec r{cls, ev(), ev(), 0.0};
+    HIPCHK(hipEventRecord(r.a, s));
+    open.push_back(r);
+    return (int)open.size() - 1;
+  }
+  void end(int id, hipStream_t s, double w) {
+    if (id < 0) return;
+    open[id].work = w;
+    HIPCHK(hipEventRecord(open[id].b, s));
+  }
+  void resolve() {  // call after the stream is synchronised
+    for (auto& r : open) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      ms[r.cls] += t;
+      launches[r.cls] += 1;
+      work[r.cls] += r.work;
+    }
+    open.clear();
+    used = 0;
+  }
+};
+Profiler g_prof;
+
+template <typename T>
+T* carve(char*& p, size_t count) {
+  T* r = reinterpret_cast<T*>(p);
+  p += ((count * sizeof(T) + 255) / 256) * 256;
+  return r;
+}
+
+struct Buffers {
+  DevState* st;
+  double* work;      // n: partial norms (permuted with the columns)
+  double* workold;   // n: norms at last recompute (positional)
+  double* topk_val;  // 2 * half
+  int* topk_idx;
+  int topk_half;
+  double* gram_part;  // nslab * 64 * 64
+  double* G;          // 64 * 64
+  double* S;          // 64 * 64 (V^T V)
+  double* T;          // 64 * 64
+  double* V;          // mpad * 64
+  int ldv;
+  double* Wp;  // nsplit * 64 * n
+  double* W;   // 64 * n
+  int* perm;   // 2 * 2 * KMAX
+  int nsplit;
+};
+
+size_t layout_buffers(int m, int n, char* base, Buffers* b) {
+  char* p = base;
+  const int mpad = ((m + 63) / 64) * 64;
+  const int nslab = (m + GRAM_ROWS - 1) / GRAM_ROWS + 1;
+  const int half = ((n + TOPK_CHUNK - 1) / TOPK_CHUNK) * KMAX + KMAX;
+  const int nsplit = 8;
+  Buffers t;
+  t.st = carve<DevState>(p, 1);
+  t.work = carve<double>(p, n);
+  t.workold = carve<double>(p, n);
+  t.topk_half = half;
+  t.topk_val = carve<double>(p, 2 * (size_t)half);
+  t.topk_idx = carve<int>(p, 2 * (size_t)half);
+  t.gram_part = carve<double>(p, (size_t)nslab * KMAX * KMAX);
+  t.G = carve<double>(p, KMAX * KMAX);
+  t.S = carve<double>(p, KMAX * KMAX);
+  t.T = carve<double>(p, KMAX * KMAX);
+  t.ldv = mpad;
+  t.V = carve<double>(p, (size_t)mpad * KMAX);
+  t.nsplit = nsplit;
+  t.Wp = carve<double>(p, (size_t)nsplit * KMAX * n);
+  t.W = carve<double>(p, (size_t)KMAX * n);
+  t.perm = carve<int>(p, 4 * KMAX);
+  if (b) *b = t;
+  return (size_t)(p - base);
+}
+
+// Index walk of permute_marked (src/dgeqrdm_work.c:149-262) with nz = 0 and
+// marks 0/1: the same two-pointer scan and placement order, recorded as a
+// sequence of position swaps.  marked has ncols entries, zero on entry and on
+// exit.
+void permute_walk(int ncols, std::vector<int>& marked, const int* sel, int nsel,
+                  std::vector<std::pair<int, int>>& swaps) {
+  swaps.clear();
+  for (int s = 0; s < nsel; s++) marked[sel[s]] = 1;
+  int jb = 0, jt = ncols - 1;
+  auto swp = [&](int p, int q) {
+    std::swap(marked[p], marked[q]);
+    swaps.emplace_back(p, q);
+  };
+  for (int s = 0; s < nsel; s++) {
+    const int jc = sel[s];
+    while (jb < jt && marked[jt] == 1) {
+      swp(jt, jb);
+      while (jb < ncols && marked[jb] != 0) jb++;
+    }
+    if (marked[jc] == 1) {
+      while (jb < ncols && marked[jb] != 0) jb++;
+      if (jc <= jb || jc < nsel) continue;
+      if (marked[jb] == 0) {
+        swp(jc, jb);
+        jb++;
+      }
+    }
+  }
+  for (int p = 0; p < nsel; p++) marked[p] = 0;  // the accepted ones now lead
+}
+
+struct Args {
+  int layout, m, n, lda, nb;
+  double* a;
+  int* jpvt;
+  double* tau;
+  int* ncols;
+  const double* thres;
+  hipStream_t s;
+  qrdm_step* trace;
+  int ntrace;
+  int* nsteps;
+};
+
+int check_args(const Args& x) {
+  const double delta = x.thres[0], tau = x.thres[1];
+  int info = 0;
+  if (x.layout != 102 && x.layout != 101) info = -1;
+  else if (x.m <= 0) info = -2;
+  else if (x.n <= 0) info = -3;
+  else if (x.lda < (x.m > 1 ? x.m : 1)) info = -5;
+  else if (delta < 0.0 || delta > 1.0) info = -9;
+  else if (tau < 0.0 || tau > 1.0) info = -9;
+  else if (x.nb <= 0) info = -10;
+  if (info == 0 && x.nb > KMAX) {
+    fprintf(stderr, "DGEQRDM: nb = %d exceeds the largest DM block of this device build (%d)\n",
+            x.nb, KMAX);
+    return -10;
+  }
+  if (info != 0) {
+    // the reference reports through xerbla_ and returns -1 (:577-581)
+    fprintf(stderr, " ** On entry to DGEQRDM parameter number %2d had an illegal value\n", -info);
+    return -1;
+  }
+  return 0;
+}
+
+Workspace& workspace_for(int m, int n, Buffers* b) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  Workspace& ws = g_ws[dev & 63];
+  size_t need = layout_buffers(m, n, nullptr, nullptr) + 256;
+  if (ws.bytes < need) {
+    if (ws.base) HIPCHK(hipFree(ws.base));
+    ws.base = nullptr;
+    ws.bytes = 0;
+    HIPCHK(hipMalloc(&ws.base, need));
+    ws.bytes = need;
+  }
+  if (!ws.pinned) {
+    ws.pinned_bytes = sizeof(DevState) + 4096;
+    HIPCHK(hipHostMalloc(&ws.pinned, ws.pinned_bytes, hipHostMallocDefault));
+  }
+  layout_buffers(m, n, ws.base, b);
+  return ws;
+}
+
+void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buffers& B,
+                hipStream_t s);
+
+int run_dm(const Args& x) {
+  const int m = x.m, n = x.n, lda = x.lda, nb = x.nb;
+  const double delta = x.thres[0], tau_dm = x.thres[1];
+  const double tol3z = sqrt(kEps);
+  hipStream_t s = x.s;
+  int enable_stop = 0;
+  double eta = 0.0;
+  if (x.ncols[0] == 1) {
+    enable_stop = 1;
+    eta = kEps * n;
+  } else if (x.ncols[0] == 2) {
+    enable_stop = 2;
+    eta = kEps * sqrt((double)n);
+  } else if (x.ncols[0] == 3) {
+    enable_stop = 3;
+    eta = x.thres[2];
+  }
+  if (x.nsteps) *x.nsteps = 0;
+  const int minmn = m < n ? m : n;
+  const int kcap = nb < KMAX ? nb : KMAX;  // this device handles blocks up to 64
+
+  Buffers B;
+  Workspace& ws = workspace_for(m, n, &B);
+  DevState* hst = reinterpret_cast<DevState*>(ws.pinned);
+  double* A = x.a;
+
+  // Fixed columns (src/dgeqrdm_work.c:594-610): jpvt != 0 columns go first.
+  int nfxd = 0;
+  std::vector<std::pair<int, int>> fixswaps;
+  for (int jj = 0; jj < n; jj++) {
+    if (x.jpvt[jj] != 0) {
+      if (jj != nfxd) {
+        fixswaps.emplace_back(jj, nfxd);
+        x.jpvt[jj] = x.jpvt[nfxd];
+        x.jpvt[nfxd] = jj + 1;
+      } else {
+        x.jpvt[jj] = jj + 1;
+      }
+      nfxd++;
+    } else {
+      x.jpvt[jj] = jj + 1;
+    }
+  }
+  for (auto& pr : fixswaps) launch_swap2(A, lda, m, pr.first, pr.second, s);
+  HIPCHK(hipMemsetAsync(B.st, 0, sizeof(DevState), s));
+  if (nfxd > 0) {
+    // Unpivoted blocked QR of the fixed columns, applied to the rest
+    // (LAPACKE_dgeqrf + LAPACKE_dormqr, src/dgeqrdm_work.c:618-635).
+    blocked_qr(A, lda, m, n, nfxd < m ? nfxd : m, x.tau, B, s);
+  }
+  int info = 0;
+  if (nfxd < minmn) {
+    int pi = g_prof.begin(PC_INIT, s);
+    launch_init_norms(A, lda, nfxd, m, n, B.work, B.workold, B.st, s);
+    g_prof.end(pi, s, 8.0 * (double)(m - nfxd) * n);
+    // columns < nfxd are not DM columns; their norms are never read
+    HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hst->flags & 1) return -13;  // non-finite input: see DESIGN notes
+    double maxnrm = 0.0;
+    {
+      unsigned long long bits = hst->maxnrm_bits;
+      memcpy(&maxnrm, &bits, sizeof(double));
+    }
+    eta = eta * maxnrm;
+    std::vector<int> marked(n, 0);
+    std::vector<std::pair<int, int>> swaps;
+    std::vector<int> content(n), where(n);
+    for (int p = 0; p < n; p++) content[p] = p;
+    int j = nfxd, it = -1;
+    while (j < minmn) {
+      const int msub = m - j, nsub = n - j;
+      it++;
+      int kmax = nb;
+      if (msub < kmax) kmax = msub;
+      if (nsub < kmax) kmax = nsub;
+      if (kmax > kcap) kmax = kcap;
+      // --- DM_perm (src/dgeqrdm_work.c:280-418) -----------------------------
+      int ps = g_prof.begin(PC_SELECT, s);
+      launch_topk(B.work + j, nsub, kmax, tau_dm, B.topk_val, B.topk_idx, B.topk_half, B.st, s);
+      // candidate Gram over rows j..m (cosines need it only when nc > 1; the
+      // kernel reads nc from the state and zero-fills beyond it)
+      launch_gram(A + (size_t)j * lda, lda, j, m, B.st->cand_idx, kmax, &B.st->nc, B.gram_part,
+                  B.G, s);
+      launch_greedy(B.G, delta, kmax, B.st, s);
+      g_prof.end(ps, s, 8.0 * nsub + 8.0 * (double)msub * kmax);
+      HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      const int fjb = hst->fjb;
+      permute_walk(nsub, marked, hst->sel, fjb, swaps);
+      // compose the swap sequence into (dst, src) pairs over touched columns
+      std::vector<int> touched;
+      for (auto& pr : swaps) {
+        for (int q : {pr.first, pr.second}) {
+          if (content[q] == q && std::find(touched.begin(), touched.end(), q) == touched.end())
+            touched.push_back(q);
+        }
+        std::swap(content[pr.first], content[pr.second]);
+        std::swap(x.jpvt[j + pr.first], x.jpvt[j + pr.second]);
+      }
+      int npair = 0;
+      int* hp = reinterpret_cast<int*>(reinterpret_cast<char*>(ws.pinned) + sizeof(DevState));
+      for (int q : touched) {
+        if (content[q] != q) {
+          hp[2 * npair] = j + q;
+          hp[2 * npair + 1] = j + content[q];
+          npair++;
+        }
+      }
+      for (int q : touched) content[q] = q;
+      if (npair > 0) {
+        // work entries are relative to column 0: shift the pair indices
+        HIPCHK(hipMemcpyAsync(B.perm, hp, sizeof(int) * 2 * npair, hipMemcpyHostToDevice, s));
+        int pp = g_prof.begin(PC_PERM, s);
+        launch_apply_perm(A, lda, m, B.work, B.perm, npair, s);
+        g_prof.end(pp, s, 16.0 * (double)m * npair);
+      }
+      // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) --------------------------
+      int pq = g_prof.begin(PC_PANEL, s);
+      launch_panel_exact(A, lda, j, msub, fjb, tau_dm, 1, x.tau + j, B.st, s);
+      g_prof.end(pq, s, 2.0 * msub * (double)fjb * fjb);
+      HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, sizeof(int), hipMemcpyDeviceToHost,
+                            s));
+      HIPCHK(hipStreamSynchronize(s));
+      const int fjb_cmp = hst->fjb_cmp;
+      x.ncols[it] = fjb_cmp;
+      if (x.trace && it < x.ntrace) {
+        qrdm_step& r = x.trace[it];
+        memset(&r, 0, sizeof(r));
+        r.j = j;
+        r.kmax = kmax;
+        r.nc = hst->nc;
+        r.fjb = fjb;
+        r.fjb_cmp = fjb_cmp;
+        r.nswap = (int)swaps.size();
+        for (int q = 0; q < fjb && q < 64; q++) r.sel[q] = hst->sel[q];
+      }
+      if (x.nsteps) *x.nsteps = it + 1;
+      // --- dlarft + dlarfb (src/dgeqrdm_work.c:751-767) ---------------------
+      int pt = g_prof.begin(PC_PANEL, s);
+      launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
+      launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
+      launch_larft(B.S, x.tau + j, B.st, B.T, s);
+      g_prof.end(pt, s, 1.0 * msub * (double)fjb_cmp * fjb_cmp);
+      const double nct = (double)(nsub - fjb);
+      int pl = g_prof.begin(PC_LARFB, s);
+      launch_larfb(A, lda, j, m, j + fjb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+      g_prof.end(pl, s, 4.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
+      // --- norm_update (src/dgeqrdm_work.c:774-778) -------------------------
+      HIPCHK(hipMemsetAsync(&B.st->maxnrm_bits, 0, sizeof(unsigned long long), s));
+      int pn = g_prof.begin(PC_NORM, s);
+      launch_norm_update(A, lda, j, m, n, B.work, B.workold, B.st, tol3z, s);
+      g_prof.end(pn, s, 8.0 * (double)(fjb_cmp + 2) * (nsub - fjb_cmp));
+      j += fjb_cmp;
+      if (enable_stop >= 1 || x.trace) {
+        HIPCHK(hipMemcpyAsync(&hst->maxnrm_bits, &B.st->maxnrm_bits, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        unsigned long long bits = hst->maxnrm_bits;
+        memcpy(&maxnrm, &bits, sizeof(double));
+        if (x.trace && it < x.ntrace) x.trace[it].maxnrm = maxnrm;
+        if (enable_stop >= 1 && maxnrm * sqrt((double)(nsub - fjb_cmp)) <= eta) break;
+      }
+    }
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  if (g_prof.on) {
+    g_prof.resolve();
+    HIPCHK(hipMemcpy(&hst->recomp_rows, &B.st->recomp_rows, sizeof(unsigned long long),
+                     hipMemcpyDeviceToHost));
+    g_prof.work[PC_NORM] += 8.0 * (double)hst->recomp_rows;
+  }
+  return info;
+}
+
+// Unpivoted blocked Householder QR (dgeqrf semantics): exact panels of up to
+// KMAX columns, each followed by its block update of the columns to the right.
+void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buffers& B,
+                hipStream_t s) {
+  for (int j = 0; j < ncols_qr; j += KMAX) {
+    const int kb = (ncols_qr - j) < KMAX ? (ncols_qr - j) : KMAX;
+    launch_panel_exact(A, lda, j, m - j, kb, 0.0, 0, tau + j, B.st, s);
+    launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
+    launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
+    launch_larft(B.S, tau + j, B.st, B.T, s);
+    launch_larfb(A, lda, j, m, j + kb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+  }
+}
+
+int run_geqrf(double* A, int lda, int m, int n, double* tau, hipStream_t s) {
+  Buffers B;
+  workspace_for(m, n, &B);
+  HIPCHK(hipMemsetAsync(B.st, 0, sizeof(DevState), s));
+  blocked_qr(A, lda, m, n, m < n ? m : n, tau, B, s);
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+// C := Q C or Q^T C, Q = H(0)...H(k-1) stored in A (m x k) and tau.
+int run_ormqr(int trans, int m, int n, int k, const double* A, int lda, const double* tau,
+              double* C, int ldc, hipStream_t s) {
+  Buffers B;
+  workspace_for(m, n, &B);
+  HIPCHK(hipMemsetAsync(B.st, 0, sizeof(DevState), s));
+  const int nblk = (k + KMAX - 1) / KMAX;
+  for (int q = 0; q < nblk; q++) {
+    const int b = trans ? q : nblk - 1 - q;  // Q^T: H(0) first; Q: H(k-1) first
+    const int j = b * KMAX;
+    const int kb = (k - j) < KMAX ? (k - j) : KMAX;
+    launch_set_kk(B.st, kb, s);
+    launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
+    launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
+    launch_larft(B.S, tau + j, B.st, B.T, s);
+    launch_larfb(C, ldc, j, m, 0, n, B.V, B.ldv, B.T, trans, B.Wp, B.W, B.nsplit, s);
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+template <typename F>
+int guarded_call(F f) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    g_err.clear();
+    return f();
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return -1000 - (int)e.e;
+  }
+}
+
+int guarded(const Args& x) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  try {
+    g_err.clear();
+    return run_dm(x);
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return -1000 - (int)e.e;
+  }
+}
+
+}  // namespace
+
+namespace {
+// Copies a column-major host block into a fresh device block (ld rounded up).
+struct DevBlock {
+  double* p = nullptr;
+  int ld = 0;
+  ~DevBlock() {
+    if (p) (void)hipFree(p);
+  }
+};
+void to_device(DevBlock& b, const double* h, int ldh, int rows, int cols) {
+  b.ld = ((rows + 31) / 32) * 32;
+  if (b.ld < 32) b.ld = 32;
+  HIPCHK(hipMalloc(&b.p, sizeof(double) * (size_t)b.ld * (cols > 0 ? cols : 1)));
+  if (rows > 0 && cols > 0)
+    HIPCHK(hipMemcpy2D(b.p, sizeof(double) * b.ld, h, sizeof(double) * ldh,
+                       sizeof(double) * rows, cols, hipMemcpyHostToDevice));
+}
+void to_host(double* h, int ldh, const DevBlock& b, int rows, int cols) {
+  if (rows > 0 && cols > 0)
+    HIPCHK(hipMemcpy2D(h, sizeof(double) * ldh, b.p, sizeof(double) * b.ld,
+                       sizeof(double) * rows, cols, hipMemcpyDeviceToHost));
+}
+}  // namespace
+
+extern "C" {
+
+const char* qrdm_version(void) { return "qrdm_amd 0.1 (gfx950)"; }
+
+void qrdm_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_prof.on = on != 0;
+}
+
+void qrdm_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int c = 0; c < kNumClasses; c++) g_prof.ms[c] = g_prof.launches[c] = g_prof.work[c] = 0;
+}
+
+int qrdm_profile_read(double* out, int nclass) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (int c = 0; c < nclass && c < kNumClasses; c++) {
+    out[3 * c] = g_prof.ms[c];
+    out[3 * c + 1] = g_prof.launches[c];
+    out[3 * c + 2] = g_prof.work[c];
+  }
+  return kNumClasses;
+}
+
+int qrdm_permute_walk(int ncols, const int* sel, int nsel, int* swaps) {
+  if (ncols <= 0 || nsel <= 0 || nsel > ncols) return -1;
+  std::vector<int> marked(ncols, 0);
+  for (int s = 0; s < nsel; s++) {
+    if (sel[s] < 0 || sel[s] >= ncols || marked[sel[s]]) return -1;
+    marked[sel[s]] = 1;
+  }
+  for (int s = 0; s < nsel; s++) marked[sel[s]] = 0;
+  std::vector<std::pair<int, int>> sw;
+  permute_walk(ncols, marked, sel, nsel, sw);
+  for (size_t i = 0; i < sw.size(); i++) {
+    swaps[2 * i] = sw[i].first;
+    swaps[2 * i + 1] = sw[i].second;
+  }
+  return (int)sw.size();
+}
+
+const char* qrdm_last_error(void) { return g_err.c_str(); }
+
+int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double* d_a, int lda, int* jpvt,
+                           double* d_tau, int* ncols, const double* thres, int nb, void* stream,
+                           qrdm_step* trace, int ntrace, int* nsteps) {
+  Args x{matrix_layout, m, n, lda, nb, d_a, jpvt, d_tau, ncols, thres,
+         (hipStream_t)stream, trace, ntrace, nsteps};
+  int bad = check_args(x);
+  if (bad) return bad;
+  return guarded(x);
+}
+
+int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double* d_a, int lda, int* jpvt,
+                     double* d_tau, int* ncols, const double* thres, int nb, void* stream) {
+  return qrdm_dgeqrdm_dev_trace(matrix_layout, m, n, d_a, lda, jpvt, d_tau, ncols, thres, nb,
+                                stream, nullptr, 0, nullptr);
+}
+
+int qrdm_dgeqrf_dev(int m, int n, double* d_a, int lda, double* d_tau, void* stream) {
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return -1;
+  if (m == 0 || n == 0) return 0;
+  return guarded_call([&] { return run_geqrf(d_a, lda, m, n, d_tau, (hipStream_t)stream); });
+}
+
+int qrdm_dormqr_dev(int trans, int m, int n, int k, const double* d_a, int lda,
+                    const double* d_tau, double* d_c, int ldc, void* stream) {
+  if (m < 0 || n < 0 || k < 0 || k > m || lda < (m > 1 ? m : 1) || ldc < (m > 1 ? m : 1))
+    return -1;
+  if (m == 0 || n == 0 || k == 0) return 0;
+  return guarded_call(
+      [&] { return run_ormqr(trans, m, n, k, d_a, lda, d_tau, d_c, ldc, (hipStream_t)stream); });
+}
+
+
+int qrdm_dgeqrf(int m, int n, double* a, int lda, double* tau) {
+  if (m < 0 || n < 0 || lda < (m > 1 ? m : 1)) return -1;
+  if (m == 0 || n == 0) return 0;
+  const int k = m < n ? m : n;
+  try {
+    DevBlock da, dt;
+    to_device(da, a, lda, m, n);
+    to_device(dt, tau, k, k, 1);
+    int info = qrdm_dgeqrf_dev(m, n, da.p, da.ld, dt.p, nullptr);
+    if (info != 0) return info;
+    to_host(a, lda, da, m, n);
+    to_host(tau, k, dt, k, 1);
+    return 0;
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return -1000 - (int)e.e;
+  }
+}
+
+int qrdm_dormqr(int trans, int m, int n, int k, const double* a, int lda, const double* tau,
+                double* c, int ldc) {
+  if (m < 0 || n < 0 || k < 0 || k > m || lda < (m > 1 ? m : 1) || ldc < (m > 1 ? m : 1))
+    return -1;
+  if (m == 0 || n == 0 || k == 0) return 0;
+  try {
+    DevBlock da, dt, dc;
+    to_device(da, a, lda, m, k);
+    to_device(dt, tau, k, k, 1);
+    to_device(dc, c, ldc, m, n);
+    int info = qrdm_dormqr_dev(trans, m, n, k, da.p, da.ld, dt.p, dc.p, dc.ld, nullptr);
+    if (info != 0) return info;
+    to_host(c, ldc, dc, m, n);
+    return 0;
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    return -1000 - (int)e.e;
+  }
+}
+
+int dgeqrdm(int matrix_layout, int m, int n, double* a, int lda, int* jpvt, double* tau,
+            int* ncols, double* thres, int nb) {
+  Args x{matrix_layout, m, n, lda, nb, a, jpvt, tau, ncols, thres, nullptr, nullptr, 0, nullptr};
+  int bad = check_args(x);
+  if (bad) return bad;
+  const int minmn = m < n ? m : n;
+  double *d_a = nullptr, *d_tau = nullptr;
+  int ldd = ((m + 31) / 32) * 32;
+  try {
+    HIPCHK(hipMalloc(&d_a, sizeof(double) * (size_t)ldd * n));
+    HIPCHK(hipMalloc(&d_tau, sizeof(double) * (size_t)minmn));
+    HIPCHK(hipMemcpy2D(d_a, sizeof(double) * ldd, a, sizeof(double) * lda, sizeof(double) * m, n,
+                       hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_tau, tau, sizeof(double) * minmn, hipMemcpyHostToDevice));
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    if (d_a) (void)hipFree(d_a);
+    if (d_tau) (void)hipFree(d_tau);
+    return -1000 - (int)e.e;
+  }
+  x.a = d_a;
+  x.lda = ldd;
+  x.tau = d_tau;
+  int info = guarded(x);
+  if (info > -1000) {
+    hipError_t e1 = hipMemcpy2D(a, sizeof(double) * lda, d_a, sizeof(double) * ldd,
+                                sizeof(double) * m, n, hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(tau, d_tau, sizeof(double) * minmn, hipMemcpyDeviceToHost);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      g_err = std::string("copy back: ") + hipGetErrorString(e1 != hipSuccess ? e1 : e2);
+      info = -1000 - (int)(e1 != hipSuccess ? e1 : e2);
+    }
+  }
+  (void)hipFree(d_a);
+  (void)hipFree(d_tau);
+  return info;
+}
+
+}  // extern "C"

@@ -1,0 +1,729 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the DM-QR library.
+//
+// Wave64 throughout.  FP64 arithmetic; the trailing update runs on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64).  Each kernel names the reference code
+// whose arithmetic it performs.
+#include "qrdm_internal.h"
+
+#include <float.h>
+#include <math.h>
+
+namespace qrdm {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// LAPACK dlamch('S') / dlamch('E'): below this |beta| dlarfg rescales.
+__device__ constexpr double kSafmin = 2.2250738585072014e-308 / 1.1102230246251565e-16;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Non-negative doubles order like their bit patterns.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Two-norm of x[0..n) by one wave; every lane returns it.  Plain sum of
+// squares, redone with scaling by max|x| when the sum is out of the range
+// where squaring is exact enough (the role of dnrm2's scaling).
+__device__ double wave_nrm2(const double* __restrict__ x, int n) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double v = x[i];
+    s = fma(v, v, s);
+  }
+  s = wave_sum(s);
+  if (isnan(s)) return s;
+  if (s >= 1e-280 && s <= 1e280) return sqrt(s);
+  double amax = 0.0;
+  for (int i = lane; i < n; i += 64) amax = fmax(amax, fabs(x[i]));
+  amax = wave_max(amax);
+  if (amax == 0.0 || isinf(amax)) return amax;
+  double r = 1.0 / amax, s2 = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double v = x[i] * r;
+    s2 = fma(v, v, s2);
+  }
+  return amax * sqrt(wave_sum(s2));
+}
+
+// Block-wide sum; red must hold blockDim.x/64 doubles.  All threads return it.
+__device__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; w++) t += red[w];
+  return t;
+}
+
+__device__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; w++) t = fmax(t, red[w]);
+  return t;
+}
+
+// Block-wide two-norm of x[0..n) (scaled fallback as in wave_nrm2).
+__device__ double block_nrm2(const double* __restrict__ x, int n, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = x[i];
+    s = fma(v, v, s);
+  }
+  s = block_sum(s, red);
+  if (isnan(s)) return s;
+  if (s >= 1e-280 && s <= 1e280) return sqrt(s);
+  double amax = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) amax = fmax(amax, fabs(x[i]));
+  amax = block_max(amax, red);
+  if (amax == 0.0 || isinf(amax)) return amax;
+  double r = 1.0 / amax, s2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = x[i] * r;
+    s2 = fma(v, v, s2);
+  }
+  return amax * sqrt(block_sum(s2, red));
+}
+
+// dlapy2 (LAPACK): sqrt(x^2 + y^2) without needless overflow.
+__device__ __forceinline__ double dlapy2(double x, double y) {
+  double xa = fabs(x), ya = fabs(y);
+  double w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == 0.0 || w > DBL_MAX) return w;
+  double q = z / w;
+  return w * sqrt(1.0 + q * q);
+}
+
+// f2c d_sign(a, b) (src/dlarfg.c:21-26): the test is b >= 0, so -0.0 is '+'.
+__device__ __forceinline__ double dsign(double a, double b) {
+  double x = fabs(a);
+  return b >= 0 ? x : -x;
+}
+
+// ---------------------------------------------------------------------------
+// Initial partial norms: work[c] = workold[c] = ||A(r0:m, c)||, max over all
+// columns (src/dgeqrdm_work.c:670-682).  One wave per column.
+__global__ __launch_bounds__(256) void k_init_norms(const double* __restrict__ A, int lda,
+                                                     int r0, int m, int n, double* work,
+                                                     double* workold, DevState* st) {
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col >= n) return;
+  double nr = wave_nrm2(A + (size_t)col * lda + r0, m - r0);
+  if ((threadIdx.x & 63) == 0) {
+    work[col] = nr;
+    workold[col] = nr;
+    if (!(nr <= DBL_MAX)) atomicOr(&st->flags, 1);
+    else atomic_max_nonneg(&st->maxnrm_bits, nr);
+  }
+}
+
+void launch_init_norms(const double* A, int lda, int r0, int m, int n, double* work,
+                       double* workold, DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_norms, dim3((n + 3) / 4), dim3(256), 0, s, A, lda, r0, m, n, work,
+                     workold, st);
+}
+
+// ---------------------------------------------------------------------------
+// Top-k of the partial norms in the order of the reference's qsort with
+// cmpStruct (src/dgeqrdm_work.c:126-142, :345): value descending, and, as
+// glibc's merge sort keeps equal keys in place, position ascending on ties.
+
+__device__ __forceinline__ bool key_before(double va, int ia, double vb, int ib) {
+  return va > vb || (va == vb && ia < ib);
+}
+
+// Bitonic sort of N (power of two) keys in LDS, descending in key_before order.
+template <int N>
+__device__ void bitonic_lds(double* v, int* ix) {
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = threadIdx.x; t < N / 2; t += blockDim.x) {
+        int lo = 2 * t - (t & (jj - 1));
+        int hi = lo + jj;
+        bool up = ((lo & k) == 0);
+        double va = v[lo], vb = v[hi];
+        int ia = ix[lo], ib = ix[hi];
+        bool sw = up ? key_before(vb, ib, va, ia) : key_before(va, ia, vb, ib);
+        if (sw) {
+          v[lo] = vb; v[hi] = va;
+          ix[lo] = ib; ix[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Stage 1: each workgroup sorts TOPK_CHUNK keys and keeps its first KMAX.
+// in_idx == nullptr means the key index is its position.
+__global__ __launch_bounds__(512) void k_topk_chunk(const double* __restrict__ in_val,
+                                                    const int* __restrict__ in_idx, int n,
+                                                    double* out_val, int* out_idx) {
+  __shared__ double v[TOPK_CHUNK];
+  __shared__ int ix[TOPK_CHUNK];
+  const int base = blockIdx.x * TOPK_CHUNK;
+  for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
+    int g = base + t;
+    if (g < n) {
+      v[t] = in_val[g];
+      ix[t] = in_idx ? in_idx[g] : g;
+    } else {
+      v[t] = -1.0;
+      ix[t] = 0x7fffffff;
+    }
+  }
+  __syncthreads();
+  bitonic_lds<TOPK_CHUNK>(v, ix);
+  for (int t = threadIdx.x; t < KMAX; t += blockDim.x) {
+    out_val[blockIdx.x * KMAX + t] = v[t];
+    out_idx[blockIdx.x * KMAX + t] = ix[t];
+  }
+}
+
+// Stage 2: sort the <= TOPK_CHUNK survivors, keep kmax, count the candidates
+// (> tau * max, src/dgeqrdm_work.c:348-351).
+__global__ __launch_bounds__(512) void k_topk_final(const double* __restrict__ in_val,
+                                                    const int* __restrict__ in_idx, int n,
+                                                    int kmax, double tau, DevState* st) {
+  __shared__ double v[TOPK_CHUNK];
+  __shared__ int ix[TOPK_CHUNK];
+  for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
+    if (t < n) {
+      v[t] = in_val[t];
+      ix[t] = in_idx ? in_idx[t] : t;
+    } else {
+      v[t] = -1.0;
+      ix[t] = 0x7fffffff;
+    }
+  }
+  __syncthreads();
+  bitonic_lds<TOPK_CHUNK>(v, ix);
+  const int nk = n < kmax ? n : kmax;
+  if (threadIdx.x < KMAX) {
+    int t = threadIdx.x;
+    st->cand_val[t] = t < nk ? v[t] : 0.0;
+    st->cand_idx[t] = t < nk ? ix[t] : -1;
+  }
+  if (threadIdx.x == 0) {
+    double lim = tau * v[0];
+    int nc = 0;
+    while (nc < nk && v[nc] > lim) nc++;
+    st->nc = nc;
+  }
+}
+
+void launch_topk(const double* keys, int n, int kmax, double tau, double* tmp_val,
+                 int* tmp_idx, int half, DevState* st, hipStream_t s) {
+  const double* cv = keys;
+  const int* ci = nullptr;
+  int cn = n;
+  double* ov = tmp_val;
+  int* oi = tmp_idx;
+  while (cn > TOPK_CHUNK) {
+    int nb = (cn + TOPK_CHUNK - 1) / TOPK_CHUNK;
+    hipLaunchKernelGGL(k_topk_chunk, dim3(nb), dim3(512), 0, s, cv, ci, cn, ov, oi);
+    cv = ov;
+    ci = oi;
+    cn = nb * KMAX;
+    // alternate between the two halves of the scratch lists
+    ov = (ov == tmp_val) ? tmp_val + half : tmp_val;
+    oi = (oi == tmp_idx) ? tmp_idx + half : tmp_idx;
+  }
+  hipLaunchKernelGGL(k_topk_final, dim3(1), dim3(512), 0, s, cv, ci, cn, kmax, tau, st);
+}
+
+// ---------------------------------------------------------------------------
+// Gram matrix of up to 64 columns over a row range: partial sums per
+// GRAM_ROWS-row slab, then a fixed-order reduction.  Used for the candidate
+// cosines (cblas_dsyrk, src/dgeqrdm_work.c:376-379) and for V^T V (dlarft).
+__global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__ base, int ld,
+                                                      int r0, int r1,
+                                                      const int* __restrict__ cols,
+                                                      int ncols_max,
+                                                      const int* __restrict__ ncols_dev,
+                                                      double* partial) {
+  __shared__ double X[KMAX * (GRAM_ROWS + 1)];
+  const int nc = ncols_dev ? *ncols_dev : ncols_max;
+  const int rs = r0 + blockIdx.x * GRAM_ROWS;
+  for (int t = threadIdx.x; t < KMAX * GRAM_ROWS; t += blockDim.x) {
+    int c = t / GRAM_ROWS, r = t % GRAM_ROWS;
+    double x = 0.0;
+    if (c < nc && rs + r < r1) {
+      int cc = cols ? cols[c] : c;
+      x = base[(size_t)cc * ld + rs + r];
+    }
+    X[c * (GRAM_ROWS + 1) + r] = x;
+  }
+  __syncthreads();
+  const int p0 = (threadIdx.x >> 4) * 4, q0 = (threadIdx.x & 15) * 4;
+  double acc[4][4] = {};
+  for (int r = 0; r < GRAM_ROWS; r++) {
+    double xp[4], xq[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      xp[a] = X[(p0 + a) * (GRAM_ROWS + 1) + r];
+      xq[a] = X[(q0 + a) * (GRAM_ROWS + 1) + r];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] = fma(xp[a], xq[b], acc[a][b]);
+  }
+  double* out = partial + (size_t)blockIdx.x * KMAX * KMAX;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) out[(p0 + a) + (q0 + b) * KMAX] = acc[a][b];
+}
+
+__global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ partial,
+                                                     int nslab, double* G) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= KMAX * KMAX) return;
+  double s = 0.0;
+  for (int b = 0; b < nslab; b++) s += partial[(size_t)b * KMAX * KMAX + e];
+  G[e] = s;
+}
+
+void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, int ncols_max,
+                 const int* ncols_dev, double* partial, double* G, hipStream_t s) {
+  int nslab = (r1 - r0 + GRAM_ROWS - 1) / GRAM_ROWS;
+  if (nslab < 1) nslab = 1;
+  hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, base, ld, r0, r1, cols,
+                     ncols_max, ncols_dev, partial);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 256), dim3(256), 0, s, partial, nslab,
+                     G);
+}
+
+// ---------------------------------------------------------------------------
+// Greedy acceptance (src/dgeqrdm_work.c:354-403): the max-norm column first,
+// then, in sorted order, every candidate whose |cos| with all earlier accepts
+// is < delta.  cos(p,q) = G(p,q) / (norm_p * norm_q) with the downdated norms
+// (the reference scales each candidate by 1/norm before its dsyrk).  One wave;
+// lane t tracks the running max |cos| of candidate t against the accepts.
+__global__ __launch_bounds__(64) void k_greedy(const double* __restrict__ G, double delta,
+                                               int kmax, DevState* st) {
+  const int t = threadIdx.x;
+  const int nc = st->nc;
+  if (t == 0) {
+    st->sel[0] = st->cand_idx[0];
+    st->selpos[0] = 0;
+  }
+  if (nc <= 1) {
+    if (t == 0) st->fjb = 1;
+    return;
+  }
+  const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
+  const double r0 = 1.0 / st->cand_val[0];
+  double run = (t < nc) ? fabs(G[0 + t * KMAX] * r0 * rt) : 0.0;
+  int nsel = 1;
+  for (int i = 1; i < nc; i++) {
+    double mi = __shfl(run, i, 64);
+    if (mi < delta && nsel < kmax) {
+      if (t == 0) {
+        st->sel[nsel] = st->cand_idx[i];
+        st->selpos[nsel] = i;
+      }
+      nsel++;
+      double ri = __shfl(rt, i, 64);
+      if (t > i && t < nc) run = fmax(run, fabs(G[i + t * KMAX] * ri * rt));
+    }
+  }
+  if (t == 0) st->fjb = nsel;
+}
+
+void launch_greedy(const double* G, double delta, int kmax, DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, s, G, delta, kmax, st);
+}
+
+// ---------------------------------------------------------------------------
+// Column permutation (the data movement of permute_marked,
+// src/dgeqrdm_work.c:149-262): perm holds npair (dst, src) pairs, a
+// permutation of the touched columns; column dst receives the old column src
+// (all m rows) and work[dst] the old work[src].  Each workgroup owns a row
+// slab, reads every source first and then writes, so no slab races another.
+constexpr int PERM_ROWS = 64;
+
+__global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
+                                                    const int* __restrict__ perm, int npair) {
+  extern __shared__ double buf[];  // npair * PERM_ROWS
+  const int rs = blockIdx.x * PERM_ROWS;
+  for (int t = threadIdx.x; t < npair * PERM_ROWS; t += blockDim.x) {
+    int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
+    if (r < m) buf[t] = A[(size_t)perm[2 * p + 1] * lda + r];
+  }
+  double wv = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x < npair) wv = work[perm[2 * threadIdx.x + 1]];
+  __syncthreads();
+  for (int t = threadIdx.x; t < npair * PERM_ROWS; t += blockDim.x) {
+    int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
+    if (r < m) A[(size_t)perm[2 * p] * lda + r] = buf[t];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < npair) work[perm[2 * threadIdx.x]] = wv;
+}
+
+void launch_apply_perm(double* A, int lda, int m, double* work, const int* perm, int npair,
+                       hipStream_t s) {
+  if (npair <= 0) return;
+  hipLaunchKernelGGL(k_apply_perm, dim3((m + PERM_ROWS - 1) / PERM_ROWS), dim3(256),
+                     npair * PERM_ROWS * sizeof(double), s, A, lda, m, work, perm, npair);
+}
+
+// ---------------------------------------------------------------------------
+// Exact Householder panel: dgeqr2_mia (src/dgeqr2.c:144-191) with dlarfg_mia
+// (src/dlarfg.c:116-185) and dlarf_ (src/dlarf.c:161-185), column by column
+// in one workgroup.  Truncation: for column i > 0 stop when ||x|| < thres,
+// thres = 5e-14, or tau_dm*|R(0,0)| once column 0 is done (tau_dm > 0).
+// truncate = 0 disables the test (unpivoted QR).
+__global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j, int msub,
+                                                      int fjb, double tau_dm, int truncate,
+                                                      double* tau, DevState* st) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  double* P = A + j + (size_t)j * lda;
+  double thres = 5e-14;
+  int i;
+  for (i = 0; i < fjb; i++) {
+    const int len = msub - i;
+    double* col = P + i + (size_t)i * lda;
+    double taui = 0.0;
+    if (len > 1) {
+      double xnorm = block_nrm2(col + 1, len - 1, red);
+      if (truncate && i > 0 && xnorm < thres) break;
+      if (xnorm != 0.0) {
+        double alpha = col[0];
+        double beta = -dsign(dlapy2(alpha, xnorm), alpha);
+        int knt = 0;
+        if (fabs(beta) < kSafmin) {
+          const double rsafmn = 1.0 / kSafmin;
+          do {
+            ++knt;
+            __syncthreads();
+            for (int r = 1 + tid; r < len; r += blockDim.x) col[r] *= rsafmn;
+            beta *= rsafmn;
+            alpha *= rsafmn;
+          } while (fabs(beta) < kSafmin);
+          __syncthreads();
+          xnorm = block_nrm2(col + 1, len - 1, red);
+          beta = -dsign(dlapy2(alpha, xnorm), alpha);
+        }
+        taui = (beta - alpha) / beta;
+        const double sc = 1.0 / (alpha - beta);
+        __syncthreads();
+        for (int r = 1 + tid; r < len; r += blockDim.x) col[r] *= sc;
+        for (int k = 0; k < knt; k++) beta *= kSafmin;
+        if (tid == 0) col[0] = beta;
+      }
+    }
+    if (tid == 0) tau[i] = taui;
+    __syncthreads();
+    if (i + 1 < fjb) {
+      const double aii = col[0];
+      if (i == 0 && tau_dm > 0.0) thres = tau_dm * fabs(aii);
+      if (taui != 0.0) {
+        for (int c = i + 1 + wave; c < fjb; c += nw) {
+          double* cc = P + i + (size_t)c * lda;
+          double s = 0.0;
+          for (int r = lane; r < len; r += 64) s = fma(r == 0 ? 1.0 : col[r], cc[r], s);
+          s = wave_sum(s);
+          const double f = -taui * s;
+          for (int r = lane; r < len; r += 64) cc[r] = fma(r == 0 ? 1.0 : col[r], f, cc[r]);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) st->fjb_cmp = i;
+}
+
+void launch_panel_exact(double* A, int lda, int j, int msub, int fjb, double tau_dm,
+                        int truncate, double* tau, DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_panel_exact, dim3(1), dim3(1024), 0, s, A, lda, j, msub, fjb, tau_dm,
+                     truncate, tau, st);
+}
+
+// ---------------------------------------------------------------------------
+// Dense copy of the block reflector V (unit lower trapezoidal, zero padded to
+// KMAX columns and to mpad rows, zero above row j) into V[r + c*ldv].
+__global__ __launch_bounds__(256) void k_build_v(const double* __restrict__ A, int lda, int j,
+                                                 int m, int mpad, const DevState* st,
+                                                 double* V, int ldv) {
+  const int kk = st->fjb_cmp;
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= mpad) return;
+  for (int c = 0; c < KMAX; c++) {
+    double v = 0.0;
+    int rr = r - j;  // row within the panel
+    if (c < kk && r < m && rr >= c) v = (rr == c) ? 1.0 : A[(size_t)(j + c) * lda + r];
+    V[(size_t)c * ldv + r] = v;
+  }
+}
+
+void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevState* st,
+                    double* V, int ldv, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_v, dim3((mpad + 255) / 256), dim3(256), 0, s, A, lda, j, m, mpad,
+                     st, V, ldv);
+}
+
+// ---------------------------------------------------------------------------
+// dlarft, DIRECT='F', STOREV='C' (call site src/dgeqrdm_work.c:751-754):
+// T(0:i,i) = -tau_i T(0:i,0:i) (V^T v_i)(0:i), T(i,i) = tau_i, from S = V^T V.
+__global__ __launch_bounds__(64) void k_larft(const double* __restrict__ S,
+                                              const double* __restrict__ tau,
+                                              const DevState* st, double* T) {
+  __shared__ double Ts[KMAX * KMAX];
+  __shared__ double tmp[KMAX];
+  const int p = threadIdx.x;
+  const int kk = st->fjb_cmp;
+  for (int c = 0; c < KMAX; c++) Ts[p + c * KMAX] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < kk; i++) {
+    const double ti = tau[i];
+    if (ti != 0.0) {
+      tmp[p] = (p < i) ? -ti * S[p + i * KMAX] : 0.0;
+      __syncthreads();
+      double s = 0.0;
+      if (p < i)
+        for (int q = p; q < i; q++) s += Ts[p + q * KMAX] * tmp[q];
+      __syncthreads();
+      if (p < i) Ts[p + i * KMAX] = s;
+      if (p == i) Ts[i + i * KMAX] = ti;
+    }
+    __syncthreads();
+  }
+  for (int c = 0; c < KMAX; c++) T[p + c * KMAX] = Ts[p + c * KMAX];
+}
+
+void launch_larft(const double* S, const double* tau, const DevState* st, double* T,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, S, tau, st, T);
+}
+
+// ---------------------------------------------------------------------------
+// Trailing update (LAPACKE_dlarfb_mia, src/dlarfb.c:142 -> dlarfb 'L','T','F',
+// 'C'): C := C - V (C^T V T)^T on rows [j, m), columns [c0, n); with
+// trans = 0 the block reflector itself, C := C - V T V^T C.
+//   gemm1:   Wp[s] = V(rows of slab s)^T C(rows of slab s)  (split over rows)
+//   wreduce: W = T^T sum_s Wp[s]
+//   gemm2:   C -= V W
+// Matrix cores: v_mfma_f64_16x16x4f64, D(16x16) += A(16x4) B(4x16); lane l
+// holds A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+v][l%16], v = 0..3.
+
+__global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
+                                               int m, int c0, int ncol,
+                                               const double* __restrict__ V, int ldv,
+                                               int slab, double* Wp, int ldw) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rs = j + blockIdx.y * slab;
+  const int re = min(rs + slab, m);
+  const int c = blockIdx.x * 64 + wave * 16 + (lane & 15);
+  const bool cok = c < ncol;
+  const double* Cc = A + (size_t)(c0 + (cok ? c : 0)) * lda;
+  dbl4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int r0 = rs; r0 < re; r0 += 4) {
+    const int r = r0 + (lane >> 4);
+    const bool rok = r < re;
+    const double b = (rok && cok) ? Cc[r] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const double a = rok ? V[(size_t)(16 * t + (lane & 15)) * ldv + r] : 0.0;
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  if (cok) {
+    double* out = Wp + (size_t)blockIdx.y * KMAX * ldw;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) out[(size_t)(16 * t + 4 * (lane >> 4) + v) * ldw + c] = acc[t][v];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_wreduce(const double* __restrict__ Wp, int nsplit,
+                                                 int ldw, const double* __restrict__ T,
+                                                 int trans, int ncol, double* W) {
+  __shared__ double Ts[KMAX * KMAX];
+  for (int t = threadIdx.x; t < KMAX * KMAX; t += blockDim.x) Ts[t] = T[t];
+  __syncthreads();
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncol) return;
+  double w[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; i++) {
+    double s = 0.0;
+    for (int b = 0; b < nsplit; b++) s += Wp[((size_t)b * KMAX + i) * ldw + c];
+    w[i] = s;
+  }
+  if (trans) {  // W = T^T w: H^T C, as dgeqrdm applies it
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i <= k; i++) s = fma(Ts[i + k * KMAX], w[i], s);
+      W[(size_t)k * ldw + c] = s;
+    }
+  } else {  // W = T w: H C (dormqr with trans = 'N')
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = k; i < KMAX; i++) s = fma(Ts[k + i * KMAX], w[i], s);
+      W[(size_t)k * ldw + c] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m, int c0,
+                                               int ncol, const double* __restrict__ V, int ldv,
+                                               const double* __restrict__ W, int ldw) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int rbase = j + blockIdx.x * 128 + wave * 32;
+  const int cbase = blockIdx.y * 64;
+  dbl4 acc[2][4];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < KMAX; k0 += 4) {
+    const int kq = k0 + (lane >> 4);
+    double a[2], b[4];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      const int r = rbase + 16 * t + (lane & 15);
+      a[t] = (r < m) ? V[(size_t)kq * ldv + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = cbase + 16 * u + (lane & 15);
+      b[u] = (c < ncol) ? W[(size_t)kq * ldw + c] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int c = cbase + 16 * u + (lane & 15);
+    if (c >= ncol) continue;
+    double* Cc = A + (size_t)(c0 + c) * lda;
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = rbase + 16 * t + 4 * (lane >> 4) + v;
+        if (r < m) Cc[r] -= acc[t][u][v];
+      }
+  }
+}
+
+void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                  const double* T, int trans, double* Wp, double* W, int nsplit,
+                  hipStream_t s) {
+  const int ncol = n - c0, mrow = m - j;
+  if (ncol <= 0 || mrow <= 0) return;
+  const int ldw = n;
+  int slab = (mrow + nsplit - 1) / nsplit;
+  slab = (slab + 3) & ~3;
+  nsplit = (mrow + slab - 1) / slab;
+  hipLaunchKernelGGL(k_gemm1, dim3((ncol + 63) / 64, nsplit), dim3(256), 0, s, A, lda, j, m, c0,
+                     ncol, V, ldv, slab, Wp, ldw);
+  hipLaunchKernelGGL(k_wreduce, dim3((ncol + 255) / 256), dim3(256), 0, s, Wp, nsplit, ldw, T,
+                     trans, ncol, W);
+  hipLaunchKernelGGL(k_gemm2, dim3((mrow + 127) / 128, (ncol + 63) / 64), dim3(256), 0, s, A,
+                     lda, j, m, c0, ncol, V, ldv, W, ldw);
+}
+
+// ---------------------------------------------------------------------------
+// norm_update (src/dgeqrdm_work.c:80-109): Drmac-Bujanovic downdate of the
+// partial norms of columns [j+kk, n) after kk rows were triangularised; when
+// temp2 <= tol3z the norm is recomputed from the rows below and workold
+// reset.  workold is indexed by position and is not permuted, as in the
+// reference.  One wave per column.
+__global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ A, int lda,
+                                                     int j, int m, int n, double* work,
+                                                     double* workold, DevState* st,
+                                                     double tol3z) {
+  const int kk = st->fjb_cmp;
+  const int col = j + kk + blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (col >= n) return;
+  double wk = work[col];
+  if (wk == 0.0) return;
+  const double* a = A + (size_t)col * lda + j;
+  const double x = (lane < kk) ? a[lane] : 0.0;
+  const double s = wave_sum(x * x);
+  double temp = sqrt(fabs(s)) / wk;
+  const double d2 = (temp + 1.0) * (1.0 - temp);
+  temp = (0.0 >= d2) ? 0.0 : d2;
+  const double d1 = wk / workold[col];
+  const double temp2 = temp * (d1 * d1);
+  const int mrem = m - j - kk;
+  if (temp2 <= tol3z) {
+    wk = (mrem > 0) ? wave_nrm2(a + kk, mrem) : 0.0;
+    if (lane == 0) {
+      workold[col] = wk;
+      atomicAdd(&st->recomp_rows, (unsigned long long)(mrem > 0 ? mrem : 0));
+    }
+  } else {
+    wk *= sqrt(temp);
+  }
+  if (lane == 0) {
+    work[col] = wk;
+    if (wk == wk) atomic_max_nonneg(&st->maxnrm_bits, wk);
+  }
+}
+
+void launch_norm_update(const double* A, int lda, int j, int m, int n, double* work,
+                        double* workold, DevState* st, double tol3z, hipStream_t s) {
+  const int ncols = n - j;  // upper bound; the kernel skips the first fjb_cmp
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL(k_norm_update, dim3((ncols + 3) / 4), dim3(256), 0, s, A, lda, j, m, n,
+                     work, workold, st, tol3z);
+}
+
+// ---------------------------------------------------------------------------
+// Small helpers.
+__global__ void k_set_kk(DevState* st, int kk) { st->fjb_cmp = kk; }
+
+void launch_set_kk(DevState* st, int kk, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_kk, dim3(1), dim3(1), 0, s, st, kk);
+}
+
+// Swap two whole columns (cblas_dswap of the fixed-column pre-pass,
+// src/dgeqrdm_work.c:599).
+__global__ __launch_bounds__(256) void k_swap2(double* A, int lda, int m, int p, int q) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= m) return;
+  double* cp = A + (size_t)p * lda;
+  double* cq = A + (size_t)q * lda;
+  const double t = cp[r];
+  cp[r] = cq[r];
+  cq[r] = t;
+}
+
+void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s) {
+  hipLaunchKernelGGL(k_swap2, dim3((m + 255) / 256), dim3(256), 0, s, A, lda, m, p, q);
+}
+
+}  // namespace qrdm

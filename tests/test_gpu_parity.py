@@ -1,0 +1,311 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle and the
+golden fixtures written by the compiled reference.
+
+Bar: jpvt, ncols and info identical; R within 1e-12 normwise; every
+reflector (V column, tau) within 1e-12 after weighting by |R_kk|/|R_00|, and
+unweighted within 1e-12 on the generic config-1 input (tests/helpers.py).
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import inputs
+from helpers import assert_parity, rel
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+ip = ctypes.POINTER(ctypes.c_int)
+dp = ctypes.POINTER(ctypes.c_double)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import qrdm_amd
+    qrdm_amd.lib()
+    return torch
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_host(a, delta=0.9, tau=0.15, nb=64, stop=0, eta=0.0, jpvt=None):
+    """DM-QR through the host-pointer C ABI (the drop-in dgeqrdm)."""
+    import qrdm_amd
+    m, n = a.shape
+    af = np.asfortranarray(a).copy(order="F")
+    jp = np.zeros(n, np.int32) if jpvt is None else np.array(jpvt, np.int32)
+    nc = np.zeros(max(n, 1), np.int32)
+    nc[0] = stop
+    t = np.zeros(min(m, n))
+    th = np.array([delta, tau, eta])
+    info = qrdm_amd.lib().dgeqrdm(102, m, n, af.ctypes.data_as(dp), m, jp.ctypes.data_as(ip),
+                                  t.ctypes.data_as(dp), nc.ctypes.data_as(ip),
+                                  th.ctypes.data_as(dp), nb)
+    k = 0
+    s = 0
+    while k < len(nc) and s < min(m, n) and nc[k] > 0:
+        s += int(nc[k])
+        k += 1
+    return dict(info=info, a=af, jpvt=jp, tau=t, ncols=nc[:k].copy())
+
+
+def run_dev(gpu, a, delta=0.9, tau=0.15, nb=64, stop=0, trace=False):
+    """DM-QR through the device-pointer C ABI (matrix resident in HBM)."""
+    import qrdm_amd
+    torch = gpu
+    m, n = a.shape
+    at = torch.from_numpy(np.ascontiguousarray(np.asfortranarray(a).T)).cuda()  # (n, m)
+    t = torch.zeros(min(m, n), dtype=torch.float64, device="cuda")
+    jp = np.zeros(n, np.int32)
+    nc = np.zeros(n, np.int32)
+    nc[0] = stop
+    info, steps = qrdm_amd.dgeqrdm_dev(at, m, t, jp, nc, np.array([delta, tau, 0.0]), nb=nb,
+                                       trace=trace)
+    torch.cuda.synchronize()
+    k = len(steps) if steps is not None else None
+    if k is None:
+        k, s = 0, 0
+        while k < n and s < min(m, n) and nc[k] > 0:
+            s += int(nc[k])
+            k += 1
+    return dict(info=info, a=np.asfortranarray(at.cpu().numpy().T), jpvt=jp,
+                tau=t.cpu().numpy(), ncols=nc[:k].copy(), steps=steps)
+
+
+# ----------------------------------------------------------------- config 1
+
+def test_config1_golden_and_oracle(gpu, oracle):
+    g = np.load(os.path.join(GOLD, "config1_ref.npz"))
+    a = inputs.gaussian(1024, 512, int(g["seed"]))
+    say("config1 1024x512 seed", int(g["seed"]))
+    got = run_host(a)
+    assert got["info"] == 0, got["info"]
+    assert np.array_equal(got["jpvt"], g["jpvt"])
+    assert np.array_equal(got["ncols"], g["ncols"])
+    assert np.abs(got["tau"] - g["tau"]).max() <= 1e-12
+    R = np.triu(got["a"][:512])
+    assert rel(np.diag(R), g["diag_r"]) <= 1e-12
+    assert rel(np.linalg.norm(R, axis=0), g["r_colnorms"]) <= 1e-12
+    assert rel(np.linalg.norm(np.tril(got["a"], -1), axis=0), g["v_colnorms"]) <= 1e-12
+    ref = oracle.dgeqrdm(a)
+    assert_parity(got, ref, strict=True)
+
+
+def test_config1_trace_step_by_step(gpu, oracle):
+    a = inputs.gaussian(1024, 512, inputs.CONFIG1_SEED)
+    got = run_dev(gpu, a, trace=True)
+    ref = oracle.dgeqrdm(a, trace=True)
+    assert len(got["steps"]) == len(ref["steps"])
+    for sg, sr in zip(got["steps"], ref["steps"]):
+        for key in ("j", "kmax", "nc", "fjb", "fjb_cmp", "nswap", "sel"):
+            assert sg[key] == sr[key], (key, sg["j"], sg[key], sr[key])
+        assert abs(sg["maxnrm"] - sr["maxnrm"]) <= 1e-12 * sr["maxnrm"]
+
+
+def test_host_and_device_entry_points_agree(gpu):
+    a = inputs.gaussian(300, 200, 99)
+    h = run_host(a, nb=32)
+    d = run_dev(gpu, a, nb=32)
+    assert np.array_equal(h["jpvt"], d["jpvt"]) and np.array_equal(h["ncols"], d["ncols"])
+    assert np.array_equal(h["a"], d["a"]) and np.array_equal(h["tau"], d["tau"])
+
+
+# ------------------------------------------------------- fixtures + sweeps
+
+def _small_cases():
+    g = np.load(os.path.join(GOLD, "small_ref.npz"))
+    out = []
+    for i in range(int(g["ncases"])):
+        m, n, nb, stop, seed = (int(x) for x in g["case%d_params" % i])
+        delta, tau = (float(x) for x in g["case%d_dt" % i])
+        ref = dict(a=g["case%d_a" % i], jpvt=g["case%d_jpvt" % i], tau=g["case%d_tau" % i],
+                   ncols=g["case%d_ncols" % i], info=int(g["case%d_info" % i]))
+        out.append((str(g["case%d_kind" % i]), m, n, nb, delta, tau, stop, seed, ref))
+    return out
+
+
+@pytest.mark.parametrize("case", _small_cases(), ids=lambda c: "%s-%dx%d-nb%d" % c[:4])
+def test_small_golden(gpu, case):
+    kind, m, n, nb, delta, tau, stop, seed, ref = case
+    say("small golden", kind, m, n, nb, delta, tau, stop, seed)
+    got = run_host(inputs.small_case(kind, m, n, seed), delta=delta, tau=tau, nb=nb, stop=stop,
+                   eta=1e-3)
+    assert_parity(got, ref)
+
+
+KINDS = ["gauss", "int", "zerocols", "dupcols", "scaled"]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_sweep_vs_oracle(gpu, oracle, seed):
+    rng = np.random.default_rng(7000 + seed)
+    kind = KINDS[seed % len(KINDS)]
+    m, n = int(rng.integers(1, 400)), int(rng.integers(1, 400))
+    if kind == "int":
+        m = max(m, 16)  # keep exactly parallel integer columns rare
+    nb = int(rng.choice([1, 2, 7, 16, 32, 64]))
+    delta = float(rng.choice([0.3, 0.6, 0.9] + ([] if kind == "int" else [1.0, 0.0])))
+    tau = float(rng.choice([0.0, 0.15, 0.5, 1.0]))
+    stop = int(rng.choice([0, 0, 1, 2]))
+    a = inputs.small_case(kind, m, n, 9000 + seed)
+    say("sweep", seed, kind, m, n, nb, delta, tau, stop)
+    got = run_host(a, delta=delta, tau=tau, nb=nb, stop=stop)
+    ref = oracle.dgeqrdm(a, delta=delta, tau=tau, nb=nb, stop=stop)
+    assert_parity(got, ref)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (2, 2), (64, 64), (65, 64),
+                                   (64, 65), (257, 129), (130, 600)])
+def test_edge_shapes(gpu, oracle, shape):
+    m, n = shape
+    a = inputs.gaussian(m, n, m * 1000 + n)
+    say("edge shape", m, n)
+    got = run_host(a)
+    ref = oracle.dgeqrdm(a)
+    assert_parity(got, ref)
+
+
+def test_zero_matrix(gpu, oracle):
+    a = np.zeros((20, 15), order="F")
+    got = run_host(a)
+    ref = oracle.dgeqrdm(a)
+    assert_parity(got, ref)
+    assert np.all(got["a"] == 0.0) and np.all(got["tau"] == 0.0)
+
+
+def test_stop_mode3_low_rank(gpu, oracle):
+    a = inputs.small_case("lowrank", 200, 150, 31)
+    got = run_host(a, stop=3, eta=1e-3)
+    ref = oracle.dgeqrdm(a, stop=3, eta=1e-3)
+    assert_parity(got, ref)
+    assert int(got["ncols"].sum()) == 50  # stops at the numerical rank
+
+
+def test_fixed_columns(gpu, oracle):
+    """jpvt != 0 on entry: those columns lead, factored without pivoting;
+    the rest by DM (corrected semantics of src/dgeqrdm_work.c:594-635)."""
+    a = inputs.gaussian(120, 90, 5)
+    jp = np.zeros(90, np.int32)
+    jp[[3, 40, 41, 77]] = 1
+    got = run_host(a, jpvt=jp, nb=16)
+    ref = oracle.dgeqrdm(a, jpvt=jp, nb=16)
+    assert_parity(got, ref)
+    assert list(got["jpvt"][:4]) == [4, 41, 42, 78]
+
+
+def test_nonfinite_input_returns_minus13(gpu):
+    a = inputs.gaussian(40, 30, 3)
+    a[5, 7] = np.nan
+    assert run_host(a)["info"] == -13
+    a[5, 7] = np.inf
+    assert run_host(a)["info"] == -13
+
+
+def test_nb_above_64_is_rejected(gpu):
+    assert run_host(inputs.gaussian(80, 80, 1), nb=65)["info"] == -10
+
+
+# ----------------------------------------------- QRF / DORMQR / Python module
+
+def test_python_module_qrdm_raw_pointer_semantics(gpu, oracle):
+    """QRDM.QRDM reads raw column-major memory: a C-ordered (m, n) array with
+    layout 102 factors its transpose, as the reference's binding does."""
+    import QRDM
+    x = np.ascontiguousarray(inputs.gaussian(50, 80, 8))  # C-ordered 50 x 80
+    m, n = 80, 50  # its memory is the column-major 80 x 50 matrix x.T
+    buf = x.copy()
+    jp = np.zeros(n, np.int32)
+    t = np.zeros(n)
+    nc = np.zeros(n, np.int32)
+    info = QRDM.QRDM(102, m, n, buf, m, jp, t, nc, np.array([0.9, 0.15]), 64)
+    ref = oracle.dgeqrdm(np.asfortranarray(x.T))
+    k = 0
+    s = 0
+    while s < n:
+        s += int(nc[k])
+        k += 1
+    got = dict(info=info, a=np.asfortranarray(buf.T), jpvt=jp, tau=t, ncols=nc[:k])
+    assert_parity(got, ref)
+
+
+def test_qrf_and_dormqr_form_q(gpu, oracle):
+    import QRDM
+    m, n = 150, 100
+    a = inputs.gaussian(m, n, 4)
+    b = a.copy(order="F")
+    t = np.zeros(n)
+    assert QRDM.QRF(102, m, n, b.T, m, t) == 0  # b.T: raw memory of b
+    q = np.asfortranarray(np.eye(m))
+    assert QRDM.DORMQR(102, m, m, n, b.T, m, t, q.T, m) == 0
+    R = np.triu(b[:n])
+    assert rel(q[:, :n] @ R, a) < 1e-14
+    assert np.linalg.norm(q.T @ q - np.eye(m)) < 1e-13
+    # same reflectors as LAPACK's dgeqrf (the reference's QRF)
+    import scipy.linalg.lapack as lapack
+    qr, tau, work, info = lapack.dgeqrf(a.copy(order="F"))
+    assert rel(np.triu(b), np.triu(qr)) < 1e-13 and np.abs(t - tau).max() < 1e-13
+
+
+def test_dormqr_of_dm_factorisation(gpu, oracle):
+    """Q from the DM reflectors (GPU dormqr) reproduces A P = Q R."""
+    import QRDM
+    m, n = 300, 220
+    a = inputs.gaussian(m, n, 12)
+    got = run_host(a)
+    q = np.asfortranarray(np.eye(m))
+    assert QRDM.DORMQR(102, m, m, n, np.ascontiguousarray(got["a"].T), m, got["tau"], q.T, m) == 0
+    R = np.triu(got["a"][:n])
+    assert rel(q[:, :n] @ R, a[:, got["jpvt"] - 1]) < 1e-14
+    assert np.linalg.norm(q.T @ q - np.eye(m)) < 1e-13
+    ref_q = oracle.form_q(got["a"], got["tau"])
+    assert rel(q[:, :n], ref_q) < 1e-13
+
+
+# --------------------------------------------------------- config 2 (full)
+
+@pytest.mark.slow
+def test_config2_low_rank_8192(gpu):
+    """8192^2 G1 G2 + 1e-10 N: leading 2048 pivots as the compiled reference
+    chose them, the |R_kk| cliff at k = 2048, and A P = Q R checked through
+    the GPU dormqr on a probe block."""
+    torch = gpu
+    import qrdm_amd
+    g = np.load(os.path.join(GOLD, "config2_ref.npz"))
+    say("config2 generating input, seed", int(g["seed"]))
+    a = inputs.lowrank_noise(8192, 8192, 2048, int(g["seed"]))
+    say("config2 factorising")
+    m = n = 8192
+    at = torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+    t = torch.zeros(n, dtype=torch.float64, device="cuda")
+    jp = np.zeros(n, np.int32)
+    nc = np.zeros(n, np.int32)
+    info, _ = qrdm_amd.dgeqrdm_dev(at, m, t, jp, nc, np.array([0.9, 0.15, 0.0]))
+    torch.cuda.synchronize()
+    assert info == 0
+    assert np.array_equal(jp[:2048], g["jpvt_lead"])
+    d = np.abs(torch.diagonal(at).cpu().numpy())
+    assert rel(d[:2048], g["diag_r"][:2048]) < 1e-12
+    assert d[2047] / d[2048] > 1e6  # numerical rank 2048
+    assert np.abs(t[:2048].cpu().numpy() - g["tau_lead"]).max() < 1e-10
+    # A P = Q R on 64 probe columns: Q R(:, probe) via dormqr
+    probe = np.random.default_rng(0).choice(n, 64, replace=False)
+    R = torch.triu(at.T)  # (m, n) view of the factored array, upper part
+    c = R[:, probe].contiguous().T.contiguous()  # (64, m): column-major m x 64
+    tq = t.clone()
+    L = qrdm_amd.lib()
+    info = L.qrdm_dormqr_dev(0, m, 64, n, ctypes.c_void_p(at.data_ptr()), m,
+                             ctypes.c_void_p(tq.data_ptr()), ctypes.c_void_p(c.data_ptr()), m, None)
+    torch.cuda.synchronize()
+    assert info == 0
+    qr = c.cpu().numpy().T
+    ap = a[:, jp[probe] - 1]
+    assert rel(qr, ap) < 1e-13
