@@ -47,7 +47,7 @@ def make_input(torch, m, n, r, seed):
     return a
 
 
-def cpu_baseline(n_sample=4096, rank_sample=1024):
+def cpu_baseline(n_sample=8192, rank_sample=2048):
     """The compiled reference (oracle/_ref) on a bounded sample of the same
     workload family; returns the dict for the JSON line or None."""
     try:

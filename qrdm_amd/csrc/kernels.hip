@@ -524,7 +524,8 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 //   wreduce: W = T^T sum_s Wp[s]
 //   gemm2:   C -= V W
 // Matrix cores: v_mfma_f64_16x16x4f64, D(16x16) += A(16x4) B(4x16); lane l
-// holds A[l%16][l/16], B[l/16][l%16] and D[4*(l/16)+v][l%16], v = 0..3.
+// holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4*v][l%16], v = 0..3 (the
+// output rows of one lane are interleaved by 4, as measured on gfx950).
 
 __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
                                                int m, int c0, int ncol,
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) out[(size_t)(16 * t + 4 * (lane >> 4) + v) * ldw + c] = acc[t][v];
+      for (int v = 0; v < 4; v++) out[(size_t)(16 * t + (lane >> 4) + 4 * v) * ldw + c] = acc[t][v];
   }
 }
 
@@ -631,7 +632,7 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
     for (int t = 0; t < 2; t++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = rbase + 16 * t + 4 * (lane >> 4) + v;
+        const int r = rbase + 16 * t + (lane >> 4) + 4 * v;
         if (r < m) Cc[r] -= acc[t][u][v];
       }
   }
