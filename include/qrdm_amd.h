@@ -61,6 +61,7 @@ typedef struct {
   int fjb;       /* columns accepted by the cosine test */
   int fjb_cmp;   /* columns reduced by the panel (<= fjb) */
   int nswap;     /* column swaps made by the permutation */
+  int panel;     /* 1: Gram-based panel, 0: column-by-column panel */
   double maxnrm; /* largest partial norm after the step's downdate */
   int sel[64];   /* accepted columns relative to j, in acceptance order */
 } qrdm_step;
@@ -68,6 +69,8 @@ typedef struct {
 /*
  * qrdm_dgeqrdm_dev plus a trace: up to ntrace records are written to trace
  * and the number of steps taken to *nsteps.  Same arithmetic as above.
+ * Setting the environment variable QRDM_PANEL=exact makes every step use the
+ * column-by-column panel (both panel forms satisfy the same parity bar).
  */
 int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
                            double *d_tau, int *ncols, const double *thres, int nb, void *stream,

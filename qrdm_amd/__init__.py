@@ -22,7 +22,7 @@ class Step(ctypes.Structure):
     """qrdm_step of include/qrdm_amd.h."""
     _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
                 ("fjb", ctypes.c_int), ("fjb_cmp", ctypes.c_int), ("nswap", ctypes.c_int),
-                ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64)]
+                ("panel", ctypes.c_int), ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64)]
 
 
 # Every entry point include/qrdm_amd.h declares.
@@ -98,7 +98,7 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
         ns = ctypes.c_int(0)
         info = L.qrdm_dgeqrdm_dev_trace(*args, tr, cap, ctypes.byref(ns))
         steps = [dict(j=t.j, kmax=t.kmax, nc=t.nc, fjb=t.fjb, fjb_cmp=t.fjb_cmp, nswap=t.nswap,
-                      maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]))
+                      panel=t.panel, maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]))
                  for t in tr[:ns.value]]
         return info, steps
     return L.qrdm_dgeqrdm_dev(*args), None

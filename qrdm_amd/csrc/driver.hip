@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -25,6 +26,12 @@ namespace {
 thread_local std::string g_err;
 
 constexpr double kEps = 1.1102230246251565e-16;  // LAPACK dlamch('E') = 2^-53
+// Panels with at least this many rows take the Gram-based path.
+constexpr int kFastPanelRows = 128;
+// Layout of the per-step upload: up to 2*KMAX (dst, src) pairs, then KMAX
+// candidate ranks of the panel columns.
+constexpr int kRankOff = 4 * KMAX;
+constexpr int kPermInts = kRankOff + KMAX;
 
 struct HipError {
   hipError_t e;
@@ -116,18 +123,22 @@ struct Buffers {
   double* G;          // 64 * 64
   double* S;          // 64 * 64 (V^T V)
   double* T;          // 64 * 64
+  double* R1;         // 64 * 64 (fast panel)
+  double* R1inv;      // 64 * 64
+  double* G2;         // 64 * 64
+  double* M;          // 64 * 64
   double* V;          // mpad * 64
   int ldv;
   double* Wp;  // nsplit * 64 * n
   double* W;   // 64 * n
-  int* perm;   // 2 * 2 * KMAX
+  int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
   int nsplit;
 };
 
 size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   char* p = base;
   const int mpad = ((m + 63) / 64) * 64;
-  const int nslab = (m + GRAM_ROWS - 1) / GRAM_ROWS + 1;
+  const int nslab = (m + 31) / 32 + 1;  // slabs of the Gram and fast-panel passes
   const int half = ((n + TOPK_CHUNK - 1) / TOPK_CHUNK) * KMAX + KMAX;
   const int nsplit = 8;
   Buffers t;
@@ -141,12 +152,16 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   t.G = carve<double>(p, KMAX * KMAX);
   t.S = carve<double>(p, KMAX * KMAX);
   t.T = carve<double>(p, KMAX * KMAX);
+  t.R1 = carve<double>(p, KMAX * KMAX);
+  t.R1inv = carve<double>(p, KMAX * KMAX);
+  t.G2 = carve<double>(p, KMAX * KMAX);
+  t.M = carve<double>(p, KMAX * KMAX);
   t.ldv = mpad;
   t.V = carve<double>(p, (size_t)mpad * KMAX);
   t.nsplit = nsplit;
   t.Wp = carve<double>(p, (size_t)nsplit * KMAX * n);
   t.W = carve<double>(p, (size_t)KMAX * n);
-  t.perm = carve<int>(p, 4 * KMAX);
+  t.perm = carve<int>(p, kPermInts);
   if (b) *b = t;
   return (size_t)(p - base);
 }
@@ -308,9 +323,12 @@ int run_dm(const Args& x) {
     eta = eta * maxnrm;
     std::vector<int> marked(n, 0);
     std::vector<std::pair<int, int>> swaps;
-    std::vector<int> content(n), where(n);
+    std::vector<int> content(n), rank_of(n, -1);
     for (int p = 0; p < n; p++) content[p] = p;
     int j = nfxd, it = -1;
+    int n_fallback = 0;
+    const char* pe = getenv("QRDM_PANEL");
+    const bool force_exact = pe && strcmp(pe, "exact") == 0;
     while (j < minmn) {
       const int msub = m - j, nsub = n - j;
       it++;
@@ -350,21 +368,42 @@ int run_dm(const Args& x) {
           npair++;
         }
       }
+      // candidate rank of each panel column (for the panel's Gram)
+      for (int q = 0; q < fjb; q++) rank_of[hst->sel[q]] = hst->selpos[q];
+      for (int q = 0; q < fjb && q < KMAX; q++) hp[kRankOff + q] = rank_of[content[q]];
+      for (int q = 0; q < fjb; q++) rank_of[hst->sel[q]] = -1;
       for (int q : touched) content[q] = q;
+      HIPCHK(hipMemcpyAsync(B.perm, hp, sizeof(int) * kPermInts, hipMemcpyHostToDevice, s));
       if (npair > 0) {
-        // work entries are relative to column 0: shift the pair indices
-        HIPCHK(hipMemcpyAsync(B.perm, hp, sizeof(int) * 2 * npair, hipMemcpyHostToDevice, s));
         int pp = g_prof.begin(PC_PERM, s);
         launch_apply_perm(A, lda, m, B.work, B.perm, npair, s);
         g_prof.end(pp, s, 16.0 * (double)m * npair);
       }
       // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) --------------------------
+      // Tall panels: Gram-based Householder reconstruction (panel.hip), which
+      // also leaves V and T for the trailing update; short or degenerate
+      // panels: the column-by-column exact panel.
+      bool fast = fjb >= 2 && msub >= kFastPanelRows && !force_exact;
       int pq = g_prof.begin(PC_PANEL, s);
-      launch_panel_exact(A, lda, j, msub, fjb, tau_dm, 1, x.tau + j, B.st, s);
+      if (fast) {
+        launch_panel_fast(A, lda, j, m, fjb, tau_dm, x.tau + j, B.G, B.perm + kRankOff,
+                          B.gram_part, B.R1, B.R1inv, B.G2, B.T, B.M, B.V, B.ldv, B.st, s);
+        HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, 4 * sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hst->flags & 2) {
+          fast = false;
+          launch_clear_flag(B.st, 2, s);
+          n_fallback++;
+        }
+      }
+      if (!fast) {
+        launch_panel_exact(A, lda, j, msub, fjb, tau_dm, 1, x.tau + j, B.st, s);
+        HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, sizeof(int), hipMemcpyDeviceToHost,
+                              s));
+        HIPCHK(hipStreamSynchronize(s));
+      }
       g_prof.end(pq, s, 2.0 * msub * (double)fjb * fjb);
-      HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, sizeof(int), hipMemcpyDeviceToHost,
-                            s));
-      HIPCHK(hipStreamSynchronize(s));
       const int fjb_cmp = hst->fjb_cmp;
       x.ncols[it] = fjb_cmp;
       if (x.trace && it < x.ntrace) {
@@ -376,18 +415,25 @@ int run_dm(const Args& x) {
         r.fjb = fjb;
         r.fjb_cmp = fjb_cmp;
         r.nswap = (int)swaps.size();
+        r.panel = fast ? 1 : 0;
         for (int q = 0; q < fjb && q < 64; q++) r.sel[q] = hst->sel[q];
       }
       if (x.nsteps) *x.nsteps = it + 1;
       // --- dlarft + dlarfb (src/dgeqrdm_work.c:751-767) ---------------------
-      int pt = g_prof.begin(PC_PANEL, s);
-      launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
-      launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
-      launch_larft(B.S, x.tau + j, B.st, B.T, s);
-      g_prof.end(pt, s, 1.0 * msub * (double)fjb_cmp * fjb_cmp);
-      const double nct = (double)(nsub - fjb);
+      if (!fast) {
+        int pt = g_prof.begin(PC_PANEL, s);
+        launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
+        launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
+        launch_larft(B.S, x.tau + j, B.st, B.T, s);
+        g_prof.end(pt, s, 1.0 * msub * (double)fjb_cmp * fjb_cmp);
+      }
+      // The exact panel already applied its reflectors to the rejected
+      // columns j+fjb_cmp..j+fjb-1 (dlarf_ inside dgeqr2_mia); the fast one
+      // leaves them to the trailing update.
+      const int c0 = fast ? j + fjb_cmp : j + fjb;
+      const double nct = (double)(n - c0);
       int pl = g_prof.begin(PC_LARFB, s);
-      launch_larfb(A, lda, j, m, j + fjb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+      launch_larfb(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
       g_prof.end(pl, s, 4.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
       // --- norm_update (src/dgeqrdm_work.c:774-778) -------------------------
       HIPCHK(hipMemsetAsync(&B.st->maxnrm_bits, 0, sizeof(unsigned long long), s));

@@ -295,11 +295,15 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
 
 __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ partial,
                                                      int nslab, double* G) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= KMAX * KMAX) return;
+  __shared__ double part[4][64];
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
   double s = 0.0;
-  for (int b = 0; b < nslab; b++) s += partial[(size_t)b * KMAX * KMAX + e];
-  G[e] = s;
+  for (int b = q; b < nslab; b += 4) s += partial[(size_t)b * KMAX * KMAX + e];
+  part[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0) G[e] = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                     (part[2][threadIdx.x] + part[3][threadIdx.x]);
 }
 
 void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, int ncols_max,
@@ -308,8 +312,7 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
   if (nslab < 1) nslab = 1;
   hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, base, ld, r0, r1, cols,
                      ncols_max, ncols_dev, partial);
-  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 256), dim3(256), 0, s, partial, nslab,
-                     G);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G);
 }
 
 // ---------------------------------------------------------------------------
@@ -562,35 +565,45 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
 __global__ __launch_bounds__(256) void k_wreduce(const double* __restrict__ Wp, int nsplit,
                                                  int ldw, const double* __restrict__ T,
                                                  int trans, int ncol, double* W) {
-  __shared__ double Ts[KMAX * KMAX];
-  for (int t = threadIdx.x; t < KMAX * KMAX; t += blockDim.x) Ts[t] = T[t];
-  __syncthreads();
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncol) return;
-  double w[KMAX];
-#pragma unroll
-  for (int i = 0; i < KMAX; i++) {
+  __shared__ double Ts[KMAX][KMAX + 1];
+  __shared__ double Ws[KMAX][KMAX + 1];  // Ws[i][c]: sum over slabs of Wp
+  const int tid = threadIdx.x;
+  const int cbase = blockIdx.x * 64;
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) Ts[e & 63][e >> 6] = T[e];
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int i = e >> 6, c = e & 63;
     double s = 0.0;
-    for (int b = 0; b < nsplit; b++) s += Wp[((size_t)b * KMAX + i) * ldw + c];
-    w[i] = s;
+    if (cbase + c < ncol)
+      for (int b = 0; b < nsplit; b++) s += Wp[((size_t)b * KMAX + i) * ldw + cbase + c];
+    Ws[i][c] = s;
   }
+  __syncthreads();
+  const int k0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
+  double acc[4][4] = {};
   if (trans) {  // W = T^T w: H^T C, as dgeqrdm applies it
+    for (int i = 0; i < KMAX; i++) {
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-      double s = 0.0;
+      for (int a = 0; a < 4; a++) {
+        const double t = (i <= k0 + a) ? Ts[i][k0 + a] : 0.0;
 #pragma unroll
-      for (int i = 0; i <= k; i++) s = fma(Ts[i + k * KMAX], w[i], s);
-      W[(size_t)k * ldw + c] = s;
+        for (int b = 0; b < 4; b++) acc[a][b] = fma(t, Ws[i][c0 + b], acc[a][b]);
+      }
     }
   } else {  // W = T w: H C (dormqr with trans = 'N')
+    for (int i = 0; i < KMAX; i++) {
 #pragma unroll
-    for (int k = 0; k < KMAX; k++) {
-      double s = 0.0;
+      for (int a = 0; a < 4; a++) {
+        const double t = (i >= k0 + a) ? Ts[k0 + a][i] : 0.0;
 #pragma unroll
-      for (int i = k; i < KMAX; i++) s = fma(Ts[k + i * KMAX], w[i], s);
-      W[(size_t)k * ldw + c] = s;
+        for (int b = 0; b < 4; b++) acc[a][b] = fma(t, Ws[i][c0 + b], acc[a][b]);
+      }
     }
   }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (cbase + c0 + b < ncol) W[(size_t)(k0 + a) * ldw + cbase + c0 + b] = acc[a][b];
 }
 
 __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m, int c0,
@@ -649,7 +662,7 @@ void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double*
   nsplit = (mrow + slab - 1) / slab;
   hipLaunchKernelGGL(k_gemm1, dim3((ncol + 63) / 64, nsplit), dim3(256), 0, s, A, lda, j, m, c0,
                      ncol, V, ldv, slab, Wp, ldw);
-  hipLaunchKernelGGL(k_wreduce, dim3((ncol + 255) / 256), dim3(256), 0, s, Wp, nsplit, ldw, T,
+  hipLaunchKernelGGL(k_wreduce, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, nsplit, ldw, T,
                      trans, ncol, W);
   hipLaunchKernelGGL(k_gemm2, dim3((mrow + 127) / 128, (ncol + 63) / 64), dim3(256), 0, s, A,
                      lda, j, m, c0, ncol, V, ldv, W, ldw);
@@ -709,6 +722,16 @@ __global__ void k_set_kk(DevState* st, int kk) { st->fjb_cmp = kk; }
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s) {
   hipLaunchKernelGGL(k_set_kk, dim3(1), dim3(1), 0, s, st, kk);
+}
+
+__global__ void k_clear_flag(DevState* st, int bits) { st->flags &= ~bits; }
+
+void launch_clear_flag(DevState* st, int bits, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, st, bits);
+}
+
+void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s) {
+  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G);
 }
 
 // Swap two whole columns (cblas_dswap of the fixed-column pre-pass,

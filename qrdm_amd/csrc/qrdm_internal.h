@@ -16,7 +16,8 @@ struct DevState {
   int nc;          // number of norm candidates
   int fjb;         // accepted columns
   int fjb_cmp;     // columns reduced by the panel
-  int flags;       // bit 0: non-finite value met; bit 1: panel fell back
+  int flags;       // bit 0: non-finite value met; bit 1: fast panel fell back
+  int k1;          // columns the fast panel's first Cholesky factored
   unsigned long long maxnrm_bits;  // max partial norm (bit pattern, >= 0)
   unsigned long long recomp_rows;  // rows read by norm recomputes (profiling)
   int sel[KMAX];        // accepted columns (relative to j), acceptance order
@@ -56,6 +57,15 @@ void launch_norm_update(const double* A, int lda, int j, int m, int n, double* w
                         double* workold, DevState* st, double tol3z, hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
+void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s);
+// Tall-panel DM QR without per-column grid reductions (panel.hip).  Gc is
+// the step's candidate Gram (64 x 64, candidate order) and rank[p] the
+// candidate rank of panel column p.
+void launch_panel_fast(double* A, int lda, int j, int m, int fjb, double tau_dm, double* tau,
+                       const double* Gc, const int* rank, double* gram_part, double* R1,
+                       double* R1inv, double* G2, double* T, double* M, double* V, int ldv,
+                       DevState* st, hipStream_t s);
+void launch_clear_flag(DevState* st, int bits, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
 
 }  // namespace qrdm

@@ -119,6 +119,58 @@ def test_host_and_device_entry_points_agree(gpu):
     assert np.array_equal(h["a"], d["a"]) and np.array_equal(h["tau"], d["tau"])
 
 
+# ------------------------------------------------------------ panel forms
+
+@pytest.fixture
+def exact_panel():
+    os.environ["QRDM_PANEL"] = "exact"
+    yield
+    del os.environ["QRDM_PANEL"]
+
+
+def test_config1_exact_panel_also_matches(gpu, oracle, exact_panel):
+    """The column-by-column panel (fallback form) meets the same bar."""
+    a = inputs.gaussian(1024, 512, inputs.CONFIG1_SEED)
+    got = run_dev(gpu, a, trace=True)
+    assert all(st["panel"] == 0 for st in got["steps"])
+    assert_parity(got, oracle.dgeqrdm(a), strict=True)
+
+
+def test_fast_panel_is_used(gpu):
+    a = inputs.gaussian(1024, 512, inputs.CONFIG1_SEED)
+    got = run_dev(gpu, a, trace=True)
+    for st in got["steps"]:
+        assert st["panel"] == (1 if st["fjb"] >= 2 and 1024 - st["j"] >= 128 else 0), st
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_truncation_inside_tall_panels(gpu, oracle, seed):
+    """Rank-deficient tall inputs: the DM truncation fires inside panels with
+    hundreds of rows (fast form), pinned against the oracle."""
+    rng = np.random.default_rng(seed)
+    m, n = int(rng.integers(300, 700)), int(rng.integers(150, 300))
+    a = inputs.small_case("lowrank", m, n, 400 + seed)
+    nb = int(rng.choice([16, 32, 64]))
+    got = run_host(a, nb=nb, stop=3, eta=1e-3)
+    ref = oracle.dgeqrdm(a, nb=nb, stop=3, eta=1e-3)
+    assert_parity(got, ref)
+    tr = run_dev(gpu, a, nb=nb, trace=True)["steps"]
+    assert any(t["fjb_cmp"] < t["fjb"] and t["panel"] == 1 for t in tr)
+
+
+def test_structured_columns_fall_back(gpu, oracle):
+    """Columns that are already reduced (unit vectors) make dlarfg's tau = 0
+    branch or tau near 2 appear; the fast form detects it and the step runs
+    the exact panel, with results still matching the oracle."""
+    a = inputs.gaussian(400, 200, 77)
+    a[:, :6] = 0.0
+    for c in range(6):
+        a[c, c] = 50.0 + c  # large norms: picked first, already triangular
+    got = run_dev(gpu, a, trace=True)
+    assert got["steps"][0]["panel"] == 0
+    assert_parity(got, oracle.dgeqrdm(a))
+
+
 # ------------------------------------------------------- fixtures + sweeps
 
 def _small_cases():
@@ -145,11 +197,13 @@ def test_small_golden(gpu, case):
 KINDS = ["gauss", "int", "zerocols", "dupcols", "scaled"]
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(32))
 def test_sweep_vs_oracle(gpu, oracle, seed):
     rng = np.random.default_rng(7000 + seed)
     kind = KINDS[seed % len(KINDS)]
     m, n = int(rng.integers(1, 400)), int(rng.integers(1, 400))
+    if seed >= 24:  # tall: the Gram-based panel form
+        m, n = int(rng.integers(500, 1500)), int(rng.integers(100, 500))
     if kind == "int":
         m = max(m, 16)  # keep exactly parallel integer columns rare
     nb = int(rng.choice([1, 2, 7, 16, 32, 64]))
