@@ -132,6 +132,7 @@ struct Buffers {
   double* Wp;  // nsplit * 64 * n
   double* W;   // 64 * n
   int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
+  unsigned* tickets;  // ceil(n/64) counters of the trailing update (zero between uses)
   int nsplit;
 };
 
@@ -162,6 +163,7 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   t.Wp = carve<double>(p, (size_t)nsplit * KMAX * n);
   t.W = carve<double>(p, (size_t)KMAX * n);
   t.perm = carve<int>(p, kPermInts);
+  t.tickets = carve<unsigned>(p, (size_t)(n + 63) / 64 + 1);
   if (b) *b = t;
   return (size_t)(p - base);
 }
@@ -250,6 +252,9 @@ Workspace& workspace_for(int m, int n, Buffers* b) {
     HIPCHK(hipHostMalloc(&ws.pinned, ws.pinned_bytes, hipHostMallocDefault));
   }
   layout_buffers(m, n, ws.base, b);
+  // The buffer offsets depend on (m, n): clear the trailing update's ticket
+  // counters, which must start every call at zero.
+  HIPCHK(hipMemset(b->tickets, 0, sizeof(unsigned) * ((size_t)(n + 63) / 64 + 1)));
   return ws;
 }
 
@@ -433,7 +438,7 @@ int run_dm(const Args& x) {
       const int c0 = fast ? j + fjb_cmp : j + fjb;
       const double nct = (double)(n - c0);
       int pl = g_prof.begin(PC_LARFB, s);
-      launch_larfb(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+      launch_larfb(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
       g_prof.end(pl, s, 4.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
       // --- norm_update (src/dgeqrdm_work.c:774-778) -------------------------
       HIPCHK(hipMemsetAsync(&B.st->maxnrm_bits, 0, sizeof(unsigned long long), s));
@@ -472,7 +477,7 @@ void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buf
     launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
     launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
     launch_larft(B.S, tau + j, B.st, B.T, s);
-    launch_larfb(A, lda, j, m, j + kb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+    launch_larfb(A, lda, j, m, j + kb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
   }
 }
 
@@ -500,7 +505,7 @@ int run_ormqr(int trans, int m, int n, int k, const double* A, int lda, const do
     launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
     launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
     launch_larft(B.S, tau + j, B.st, B.T, s);
-    launch_larfb(C, ldc, j, m, 0, n, B.V, B.ldv, B.T, trans, B.Wp, B.W, B.nsplit, s);
+    launch_larfb(C, ldc, j, m, 0, n, B.V, B.ldv, B.T, trans, B.Wp, B.W, B.nsplit, B.tickets, s);
   }
   HIPCHK(hipStreamSynchronize(s));
   return 0;

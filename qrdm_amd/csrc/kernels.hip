@@ -530,142 +530,281 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4*v][l%16], v = 0..3 (the
 // output rows of one lane are interleaved by 4, as measured on gfx950).
 
-__global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
-                                               int m, int c0, int ncol,
-                                               const double* __restrict__ V, int ldv,
-                                               int slab, double* Wp, int ldw) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rs = j + blockIdx.y * slab;
-  const int re = min(rs + slab, m);
-  const int c = blockIdx.x * 64 + wave * 16 + (lane & 15);
-  const bool cok = c < ncol;
-  const double* Cc = A + (size_t)(c0 + (cok ? c : 0)) * lda;
-  dbl4 acc[4];
+// k_gemm1: W = op(T) (V^T C) with the reduction over rows split in slabs.
+// Rows are walked in aligned 32-row chunks; within a chunk the MFMA k index
+// of lane group g = lane/16 at step s is row 8g + s, so every lane reads 8
+// consecutive rows of one column (two 32-byte loads) for both operands.  The
+// V chunk (32 rows x 64 columns) is shared by the 4 waves through LDS.  Each
+// workgroup writes its slab's partial; the last one of a column tile (atomic
+// ticket) sums the partials in slab order and applies T^T (or T).
+constexpr int G1_CHUNK = 32;
+
+__device__ __forceinline__ void load_rows8(const double* __restrict__ col, int r, int rlo, int rhi,
+                                           double (&out)[8]) {
+  if (r >= rlo && r + 8 <= rhi && ((reinterpret_cast<uintptr_t>(col + r) & 31) == 0)) {
+    const dbl4 x = *reinterpret_cast<const dbl4*>(col + r);
+    const dbl4 y = *reinterpret_cast<const dbl4*>(col + r + 4);
+    out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; out[3] = x[3];
+    out[4] = y[0]; out[5] = y[1]; out[6] = y[2]; out[7] = y[3];
+  } else {
 #pragma unroll
-  for (int t = 0; t < 4; t++) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  for (int r0 = rs; r0 < re; r0 += 4) {
-    const int r = r0 + (lane >> 4);
-    const bool rok = r < re;
-    const double b = (rok && cok) ? Cc[r] : 0.0;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const double a = rok ? V[(size_t)(16 * t + (lane & 15)) * ldv + r] : 0.0;
-      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-    }
-  }
-  if (cok) {
-    double* out = Wp + (size_t)blockIdx.y * KMAX * ldw;
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) out[(size_t)(16 * t + (lane >> 4) + 4 * v) * ldw + c] = acc[t][v];
+    for (int q = 0; q < 8; q++) out[q] = (r + q >= rlo && r + q < rhi) ? col[r + q] : 0.0;
   }
 }
 
-__global__ __launch_bounds__(256) void k_wreduce(const double* __restrict__ Wp, int nsplit,
-                                                 int ldw, const double* __restrict__ T,
-                                                 int trans, int ncol, double* W) {
+__device__ __forceinline__ void load_rows4(const double* __restrict__ col, int r, int rlo, int rhi,
+                                           double (&out)[4]) {
+  if (r >= rlo && r + 4 <= rhi && ((reinterpret_cast<uintptr_t>(col + r) & 31) == 0)) {
+    const dbl4 x = *reinterpret_cast<const dbl4*>(col + r);
+    out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; out[3] = x[3];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = (r + q >= rlo && r + q < rhi) ? col[r + q] : 0.0;
+  }
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
+                                               int m, int c0, int ncol,
+                                               const double* __restrict__ V, int ldv,
+                                               int slab, double* Wp, int ldw,
+                                               const double* __restrict__ T, int trans,
+                                               double* W, unsigned* tickets) {
   __shared__ double Ts[KMAX][KMAX + 1];
-  __shared__ double Ws[KMAX][KMAX + 1];  // Ws[i][c]: sum over slabs of Wp
-  const int tid = threadIdx.x;
-  const int cbase = blockIdx.x * 64;
+  __shared__ double Ws[KMAX][KMAX + 1];
+  __shared__ int s_last;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, li = lane & 15;
+  const int base = j & ~(G1_CHUNK - 1);
+  const int rs = base + blockIdx.y * slab;
+  const int re = min(rs + slab, m);
+  const int rlo = max(rs, j);
+  const int cb = blockIdx.x * 64;
+  // every wave accumulates the whole 64 x 64 tile over chunks wave, wave+4,
+  // ...  Columns past ncol read a valid column instead: their output columns
+  // are never stored.
+  const double* Cc[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int c = cb + 16 * u + li;
+    Cc[u] = A + (size_t)(c0 + (c < ncol ? c : ncol - 1)) * lda;
+  }
+  const double* Vt[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) Vt[t] = V + (size_t)(16 * t + li) * ldv;
+  dbl4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  // 16-row chunks, wave w taking chunks w, w+4, ...; lane group g holds rows
+  // 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  Every chunk issues
+  // the same eight 32-byte loads (row index clamped into the column); rows
+  // outside [rlo, re) are zeroed when used.  With a fixed load count per
+  // chunk the next chunk's loads stay in flight during this chunk's MFMAs.
+  constexpr int CH = 16, STRIDE = 4 * CH;
+  // last 4-row group that starts inside the column: with lda % 4 == 0 (the
+  // vector path's requirement) rows up to round_up(m, 4) - 1 exist in memory,
+  // and a group holding any row < m never needs clamping
+  const int rclamp = ((m + 3) & ~3) - 4;
+  auto load_chunk = [&](int r, dbl4 (&cc)[4], dbl4 (&vc)[4]) {
+    if constexpr (VEC) {
+      const int rr = min(r + 4 * g, rclamp);
+#pragma unroll
+      for (int u = 0; u < 4; u++) cc[u] = *reinterpret_cast<const dbl4*>(Cc[u] + rr);
+#pragma unroll
+      for (int t = 0; t < 4; t++) vc[t] = *reinterpret_cast<const dbl4*>(Vt[t] + rr);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int rq = min(r + 4 * g + q, m - 1);
+#pragma unroll
+        for (int u = 0; u < 4; u++) cc[u][q] = Cc[u][rq];
+#pragma unroll
+        for (int t = 0; t < 4; t++) vc[t][q] = Vt[t][rq];
+      }
+    }
+  };
+  auto mfma_chunk = [&](int r, const dbl4 (&cc)[4], const dbl4 (&vc)[4]) {
+    const int rr = r + 4 * g;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const bool ok = rr + q >= rlo && rr + q < re;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const double a = ok ? vc[t][q] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const double b = ok ? cc[u][q] : 0.0;
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t][u], 0, 0, 0);
+        }
+      }
+    }
+  };
+  // chunks are processed in pairs with a fixed trip count; an odd count runs
+  // one fully masked chunk (all its rows are >= re and contribute zero)
+  dbl4 ca[4], va[4], cn[4], vn[4];
+  int r0 = rs + CH * wave;
+  const int npair = r0 < re ? (re - r0 + 2 * STRIDE - 1) / (2 * STRIDE) : 0;
+  if (npair > 0) load_chunk(r0, ca, va);
+  for (int it = 0; it < npair; it++) {
+    load_chunk(r0 + STRIDE, cn, vn);
+    mfma_chunk(r0, ca, va);
+    load_chunk(r0 + 2 * STRIDE, ca, va);
+    mfma_chunk(r0 + STRIDE, cn, vn);
+    r0 += 2 * STRIDE;
+  }
+  // sum the four waves' tiles in wave order: row (V column) 16t + g + 4v,
+  // column 16u + li
+  for (int w = 0; w < 4; w++) {
+    if (wave == w) {
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            double& d = Ws[16 * t + g + 4 * v][16 * u + li];
+            d = (w == 0) ? acc[t][u][v] : d + acc[t][u][v];
+          }
+    }
+    __syncthreads();
+  }
+  {
+    double* out = Wp + (size_t)blockIdx.y * KMAX * ldw;
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int i = e >> 6, cc = cb + (e & 63);
+      if (cc < ncol) out[(size_t)i * ldw + cc] = Ws[i][e & 63];
+    }
+  }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) s_last = (atomicAdd(&tickets[blockIdx.x], 1u) == gridDim.y - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // last workgroup of this column tile: sum the partials in slab order into
+  // LDS (each element read once, coalesced), then apply op(T)
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) Ts[e & 63][e >> 6] = T[e];
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int i = e >> 6, c = e & 63;
-    double s = 0.0;
-    if (cbase + c < ncol)
-      for (int b = 0; b < nsplit; b++) s += Wp[((size_t)b * KMAX + i) * ldw + cbase + c];
-    Ws[i][c] = s;
+    const int i = e >> 6, cc = cb + (e & 63);
+    double sum = 0.0;
+    if (cc < ncol)
+      for (int y = 0; y < (int)gridDim.y; y++) sum += Wp[((size_t)y * KMAX + i) * ldw + cc];
+    Ws[i][e & 63] = sum;
   }
   __syncthreads();
-  const int k0 = 4 * (tid >> 4), c0 = 4 * (tid & 15);
-  double acc[4][4] = {};
-  if (trans) {  // W = T^T w: H^T C, as dgeqrdm applies it
-    for (int i = 0; i < KMAX; i++) {
+  const int k0 = 4 * (tid >> 4), q0 = 4 * (tid & 15);
+  double out[4][4] = {};
+  for (int i = 0; i < KMAX; i++) {
+    double wi[4], tv[4];
 #pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const double t = (i <= k0 + a) ? Ts[i][k0 + a] : 0.0;
+    for (int b = 0; b < 4; b++) wi[b] = Ws[i][q0 + b];
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = fma(t, Ws[i][c0 + b], acc[a][b]);
-      }
+    for (int a = 0; a < 4; a++) {
+      const int k = k0 + a;
+      tv[a] = trans ? ((i <= k) ? Ts[i][k] : 0.0) : ((i >= k) ? Ts[k][i] : 0.0);
     }
-  } else {  // W = T w: H C (dormqr with trans = 'N')
-    for (int i = 0; i < KMAX; i++) {
 #pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const double t = (i >= k0 + a) ? Ts[k0 + a][i] : 0.0;
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = fma(t, Ws[i][c0 + b], acc[a][b]);
-      }
-    }
+      for (int b = 0; b < 4; b++) out[a][b] = fma(tv[a], wi[b], out[a][b]);
   }
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++)
-      if (cbase + c0 + b < ncol) W[(size_t)(k0 + a) * ldw + cbase + c0 + b] = acc[a][b];
+      if (cb + q0 + b < ncol) W[(size_t)(k0 + a) * ldw + cb + q0 + b] = out[a][b];
+  if (tid == 0) tickets[blockIdx.x] = 0u;
 }
 
+// k_gemm2: C -= V W computed as C^T -= W^T V^T on the matrix cores, so that
+// the accumulator's 16 lanes of one register hold 16 consecutive rows of a
+// column of C (coalesced loads and stores).  Workgroup tile 64 rows x 64
+// columns; wave w owns columns 16w..16w+15; the V tile (64 rows x 64) is
+// shared through LDS.
 __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m, int c0,
                                                int ncol, const double* __restrict__ V, int ldv,
                                                const double* __restrict__ W, int ldw) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int rbase = j + blockIdx.x * 128 + wave * 32;
-  const int cbase = blockIdx.y * 64;
-  dbl4 acc[2][4];
+  __shared__ double Vs[64][KMAX + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, li = lane & 15;
+  const int rbase = j + blockIdx.x * 64;
+  const int cbase = blockIdx.y * 64 + wave * 16;
+  // C values this lane updates: rows rbase + 16t + li, columns cbase + g + 4v
+  double cval[4][4];
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+  for (int v = 0; v < 4; v++) {
+    const int cc = cbase + g + 4 * v;
+    const double* col = A + (size_t)(c0 + (cc < ncol ? cc : 0)) * lda;
 #pragma unroll
-    for (int u = 0; u < 4; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < KMAX; k0 += 4) {
-    const int kq = k0 + (lane >> 4);
-    double a[2], b[4];
-#pragma unroll
-    for (int t = 0; t < 2; t++) {
-      const int r = rbase + 16 * t + (lane & 15);
-      a[t] = (r < m) ? V[(size_t)kq * ldv + r] : 0.0;
+    for (int t = 0; t < 4; t++) {
+      const int r = rbase + 16 * t + li;
+      cval[t][v] = (cc < ncol && r < m) ? col[r] : 0.0;
     }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int c = cbase + 16 * u + (lane & 15);
-      b[u] = (c < ncol) ? W[(size_t)kq * ldw + c] : 0.0;
-    }
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b[u], acc[t][u], 0, 0, 0);
   }
+  for (int e = tid; e < 64 * KMAX; e += blockDim.x) {
+    const int k = e >> 6, r = e & 63;
+    Vs[r][k] = (rbase + r < m) ? V[(size_t)k * ldv + rbase + r] : 0.0;
+  }
+  __syncthreads();
+  dbl4 acc[4];
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int c = cbase + 16 * u + (lane & 15);
-    if (c >= ncol) continue;
-    double* Cc = A + (size_t)(c0 + c) * lda;
+  for (int t = 0; t < 4; t++) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  const int wc = cbase + li;
+  const bool wok = wc < ncol;
+#pragma unroll 4
+  for (int k0 = 0; k0 < KMAX; k0 += 4) {
+    const double a = wok ? W[(size_t)(k0 + g) * ldw + wc] : 0.0;
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int t = 0; t < 4; t++) {
+      const double b = Vs[16 * t + li][k0 + g];
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  // acc[t][v]: row (of C^T) = column cbase + g + 4v, column = row rbase + 16t + li
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = rbase + 16 * t + (lane >> 4) + 4 * v;
-        if (r < m) Cc[r] -= acc[t][u][v];
-      }
+  for (int v = 0; v < 4; v++) {
+    const int cc = cbase + g + 4 * v;
+    if (cc >= ncol) continue;
+    double* col = A + (size_t)(c0 + cc) * lda;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int r = rbase + 16 * t + li;
+      if (r < m) col[r] = cval[t][v] - acc[t][v];
+    }
   }
 }
 
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                   const double* T, int trans, double* Wp, double* W, int nsplit,
-                  hipStream_t s) {
+                  unsigned* tickets, hipStream_t s) {
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ldw = n;
-  int slab = (mrow + nsplit - 1) / nsplit;
-  slab = (slab + 3) & ~3;
-  nsplit = (mrow + slab - 1) / slab;
-  hipLaunchKernelGGL(k_gemm1, dim3((ncol + 63) / 64, nsplit), dim3(256), 0, s, A, lda, j, m, c0,
-                     ncol, V, ldv, slab, Wp, ldw);
-  hipLaunchKernelGGL(k_wreduce, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, nsplit, ldw, T,
-                     trans, ncol, W);
-  hipLaunchKernelGGL(k_gemm2, dim3((mrow + 127) / 128, (ncol + 63) / 64), dim3(256), 0, s, A,
-                     lda, j, m, c0, ncol, V, ldv, W, ldw);
+  const int ctiles = (ncol + 63) / 64;
+  // enough workgroups to fill the chip: split the rows when there are few
+  // column tiles
+  int want = (1024 + ctiles - 1) / ctiles;
+  if (want > nsplit) want = nsplit;
+  const int base = j & ~(G1_CHUNK - 1);
+  const int span = m - base;
+  int slab = (span + want - 1) / want;
+  slab = ((slab + G1_CHUNK - 1) / G1_CHUNK) * G1_CHUNK;
+  if (slab < G1_CHUNK) slab = G1_CHUNK;
+  const int ns = (span + slab - 1) / slab;
+  // 32-byte vector loads need 32-byte aligned columns of at least 4 rows
+  const int aligned = (lda % 4 == 0) && (ldv % 4 == 0) && m >= 4 &&
+                      ((reinterpret_cast<uintptr_t>(A) & 31) == 0) &&
+                      ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
+  if (aligned)
+    hipLaunchKernelGGL(k_gemm1<true>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol, V,
+                       ldv, slab, Wp, ldw, T, trans, W, tickets);
+  else
+    hipLaunchKernelGGL(k_gemm1<false>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
+                       V, ldv, slab, Wp, ldw, T, trans, W, tickets);
+  hipLaunchKernelGGL(k_gemm2, dim3((mrow + 63) / 64, ctiles), dim3(256), 0, s, A, lda, j, m, c0,
+                     ncol, V, ldv, W, ldw);
 }
 
 // ---------------------------------------------------------------------------

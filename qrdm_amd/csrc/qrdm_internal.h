@@ -52,7 +52,7 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // trans = 0: C := (I - V T V^T) C.
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                   const double* T, int trans, double* Wp, double* W, int nsplit,
-                  hipStream_t s);
+                  unsigned* tickets, hipStream_t s);
 void launch_norm_update(const double* A, int lda, int j, int m, int n, double* work,
                         double* workold, DevState* st, double tol3z, hipStream_t s);
 
