@@ -28,6 +28,11 @@ METRIC = ("DM-QR time & TF/s, 8192² FP64; mixed/learned vs plain at "
           "||AP-QR||/||A||≤1e-12")
 FP64_MFMA_PEAK_TFS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_gemm1 launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
+# counter's reading for a plain copy of known size), averaged over the launches
+# of one factorisation of the workload (m, n, rank, nb); measured offline with
+# rocprofv3 --pmc, summary in profiles/k_gemm1_8192_pmc_traffic.txt.
+GEMM1_TRAFFIC = {}
 
 
 def flops(m, n):
@@ -136,13 +141,23 @@ def main():
     ms = 1000.0 * total / args.steps
     tfs = flops(m, n) / (ms / 1000.0) / 1e12
 
-    ms_l, launches_l, work_l = prof["larfb"]
-    achieved = work_l / (ms_l / 1000.0) / 1e12 if ms_l > 0 else 0.0
+    # dominant kernel: k_gemm1 (W = T^T V^T C), FP64 MFMA bound; its partner
+    # k_gemm2 (C -= V W) is HBM bound and reported beside it
+    ms_w, launches_w, flops_w = prof["larfb_w"]
+    ms_a, launches_a, bytes_a = prof["larfb_apply"]
+    achieved = flops_w / (ms_w / 1000.0) / 1e12 if ms_w > 0 else 0.0
+    avg_w = ms_w / max(launches_w, 1)
+    traffic = GEMM1_TRAFFIC.get((m, n, args.rank, args.nb))
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFS,
                 "unit": "TF/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4),
-                "traffic": None,
-                "kernel": "trailing update (k_gemm1 + k_wreduce + k_gemm2), %d launches, "
-                          "avg %.4f ms" % (launches_l, ms_l / max(launches_l, 1))}
+                "traffic": traffic,
+                "kernel": "k_gemm1 (trailing update W = T^T V^T C), %d launches, avg %.4f ms"
+                          % (launches_w, avg_w)}
+    achieved_a = bytes_a / (ms_a / 1000.0) / 1e9 if ms_a > 0 else 0.0
+    roofline_apply = {"bound": "hbm", "achieved": round(achieved_a, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(achieved_a / HBM_PEAK_GBS, 4),
+                      "kernel": "k_gemm2 (trailing update C -= V W), %d launches, avg %.4f ms"
+                                % (launches_a, ms_a / max(launches_a, 1))}
     others = {}
     for c in ("norm_update", "select", "panel", "permute", "init_norms"):
         t, k, w = prof[c]
@@ -159,9 +174,10 @@ def main():
                                    "tau=0.15 nb=%d, matrix resident in HBM"
                                    % (m, n, args.rank, args.nb),
                        "m": m, "n": n, "rank": args.rank, "nb": args.nb,
-                       "dm_steps": nsteps_dm, "larfb_ms_per_step": round(ms_l / args.steps, 3),
+                       "dm_steps": nsteps_dm,
+                       "trailing_update_ms_per_step": round((ms_w + ms_a) / args.steps, 3),
                        "kernel_classes": others},
-            "roofline": roofline}
+            "roofline": roofline, "roofline_apply": roofline_apply}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

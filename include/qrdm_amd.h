@@ -113,12 +113,13 @@ int qrdm_permute_walk(int ncols, const int *sel, int nsel, int *swaps);
  * HIP events (on the stream it launches on) around each class of kernel and
  * accumulates, per class, the summed event time in ms, the launch count and
  * the algorithmic work of those launches.  Classes:
- *   0 trailing update (larfb: V^T C, T^T W, C - V W)   work = flops
+ *   0 trailing update, first half (W = T^T V^T C)      work = flops
  *   1 norm downdate/recompute                          work = HBM bytes
  *   2 DM selection (top-k, candidate Gram, greedy)     work = HBM bytes
  *   3 panel (Householder + V + T)                      work = flops
  *   4 column permutation                               work = HBM bytes
  *   5 initial norms                                    work = HBM bytes
+ *   6 trailing update, second half (C -= V W)          work = HBM bytes
  * qrdm_profile_read fills out[3*c + {0,1,2}] = {ms, launches, work} for
  * c < nclass and returns the number of classes.  No counterpart in the
  * reference (it has no timing facility).

@@ -25,7 +25,7 @@
  * BLAS/LAPACK (setup_QRDM.py:13,38); the compiled reference used to pin this
  * file links OpenBLAS 0.3.28 as bundled with scipy 1.15.3.
  *
- * Pinning (tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle.py, tests/test_abi.py):
  *   - whole driver: against the reference compiled from its own sources
  *     (oracle/_ref, built by oracle/build_ref.sh), jpvt/ncols identical and
  *     R, V, tau within 1e-12 relative, on committed seeded fixtures;

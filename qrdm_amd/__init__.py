@@ -31,7 +31,8 @@ EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrf
            "qrdm_profile_enable", "qrdm_profile_reset", "qrdm_profile_read",
            "qrdm_last_error", "qrdm_version")
 
-PROFILE_CLASSES = ("larfb", "norm_update", "select", "panel", "permute", "init_norms")
+PROFILE_CLASSES = ("larfb_w", "norm_update", "select", "panel", "permute", "init_norms",
+                   "larfb_apply")
 
 
 def lib():

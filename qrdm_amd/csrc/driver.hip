@@ -57,8 +57,9 @@ Workspace g_ws[64];
 
 // Kernel-time accounting (qrdm_profile_*): HIP events around each class of
 // kernels on the launch stream, resolved after the call's final sync.
-constexpr int kNumClasses = 6;
-enum { PC_LARFB = 0, PC_NORM = 1, PC_SELECT = 2, PC_PANEL = 3, PC_PERM = 4, PC_INIT = 5 };
+constexpr int kNumClasses = 7;
+enum { PC_LARFB = 0, PC_NORM = 1, PC_SELECT = 2, PC_PANEL = 3, PC_PERM = 4, PC_INIT = 5,
+       PC_APPLY = 6 };
 struct Profiler {
   bool on = false;
   double ms[kNumClasses] = {}, launches[kNumClasses] = {}, work[kNumClasses] = {};
@@ -319,7 +320,11 @@ int run_dm(const Args& x) {
     // columns < nfxd are not DM columns; their norms are never read
     HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    if (hst->flags & 1) return -13;  // non-finite input: see DESIGN notes
+    // A NaN or infinity makes the reference's LAPACKE_dlarfb_mia NaN scan
+    // return -13 after parts of A were already overwritten
+    // (src/dlarfb.c:54-127); here it is caught by the norm sweep before any
+    // write, and A is left as it was.
+    if (hst->flags & 1) return -13;
     double maxnrm = 0.0;
     {
       unsigned long long bits = hst->maxnrm_bits;
@@ -438,8 +443,11 @@ int run_dm(const Args& x) {
       const int c0 = fast ? j + fjb_cmp : j + fjb;
       const double nct = (double)(n - c0);
       int pl = g_prof.begin(PC_LARFB, s);
-      launch_larfb(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
-      g_prof.end(pl, s, 4.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
+      launch_larfb_w(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
+      g_prof.end(pl, s, 2.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
+      int pa = g_prof.begin(PC_APPLY, s);
+      launch_larfb_apply(A, lda, j, m, c0, n, B.V, B.ldv, B.W, s);
+      g_prof.end(pa, s, 16.0 * msub * nct + 8.0 * msub * fjb_cmp + 8.0 * fjb_cmp * nct);
       // --- norm_update (src/dgeqrdm_work.c:774-778) -------------------------
       HIPCHK(hipMemsetAsync(&B.st->maxnrm_bits, 0, sizeof(unsigned long long), s));
       int pn = g_prof.begin(PC_NORM, s);

@@ -776,9 +776,9 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
   }
 }
 
-void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                  const double* T, int trans, double* Wp, double* W, int nsplit,
-                  unsigned* tickets, hipStream_t s) {
+void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                    const double* T, int trans, double* Wp, double* W, int nsplit,
+                    unsigned* tickets, hipStream_t s) {
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ldw = n;
@@ -803,8 +803,22 @@ void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double*
   else
     hipLaunchKernelGGL(k_gemm1<false>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
                        V, ldv, slab, Wp, ldw, T, trans, W, tickets);
+}
+
+void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                        const double* W, hipStream_t s) {
+  const int ncol = n - c0, mrow = m - j;
+  if (ncol <= 0 || mrow <= 0) return;
+  const int ctiles = (ncol + 63) / 64;
   hipLaunchKernelGGL(k_gemm2, dim3((mrow + 63) / 64, ctiles), dim3(256), 0, s, A, lda, j, m, c0,
-                     ncol, V, ldv, W, ldw);
+                     ncol, V, ldv, W, n);
+}
+
+void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                  const double* T, int trans, double* Wp, double* W, int nsplit,
+                  unsigned* tickets, hipStream_t s) {
+  launch_larfb_w(A, lda, j, m, c0, n, V, ldv, T, trans, Wp, W, nsplit, tickets, s);
+  launch_larfb_apply(A, lda, j, m, c0, n, V, ldv, W, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -53,6 +53,12 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                   const double* T, int trans, double* Wp, double* W, int nsplit,
                   unsigned* tickets, hipStream_t s);
+// the two halves of launch_larfb: W = op(T) V^T C, then C -= V W
+void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                    const double* T, int trans, double* Wp, double* W, int nsplit,
+                    unsigned* tickets, hipStream_t s);
+void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                        const double* W, hipStream_t s);
 void launch_norm_update(const double* A, int lda, int j, int m, int n, double* work,
                         double* workold, DevState* st, double tol3z, hipStream_t s);
 
