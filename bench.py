@@ -32,7 +32,24 @@ HBM_PEAK_GBS = 8000.0
 # counter's reading for a plain copy of known size), averaged over the launches
 # of one factorisation of the workload (m, n, rank, nb); measured offline with
 # rocprofv3 --pmc, summary in profiles/k_gemm1_8192_pmc_traffic.txt.
-GEMM1_TRAFFIC = {}
+GEMM1_TRAFFIC = {(8192, 8192, 2048, 64): 260096442}
+
+
+def gemm1_algorithmic_bytes(m, n, ncols):
+    """Mean algorithmic HBM bytes of one k_gemm1 launch over a factorisation
+    whose DM steps reduced ncols[it] columns: C (m_sub x (n - c0)) read once,
+    V (m_sub x k) and W (k x (n - c0)) once, c0 = j + k."""
+    j, tot, cnt = 0, 0.0, 0
+    for k in ncols:
+        k = int(k)
+        if k <= 0:
+            break
+        msub, nct = m - j, n - j - k
+        if nct > 0 and msub > 0:
+            tot += 8.0 * (msub * nct + msub * k + k * nct)
+            cnt += 1
+        j += k
+    return tot / max(cnt, 1)
 
 
 def flops(m, n):
@@ -137,6 +154,7 @@ def main():
         torch.cuda.synchronize()
         total += time.perf_counter() - t0
         nsteps_dm = int(np.count_nonzero(nc))
+        last_nc = nc
     prof = qrdm_amd.profile(enable=False, reset=True)
     ms = 1000.0 * total / args.steps
     tfs = flops(m, n) / (ms / 1000.0) / 1e12
@@ -148,9 +166,10 @@ def main():
     achieved = flops_w / (ms_w / 1000.0) / 1e12 if ms_w > 0 else 0.0
     avg_w = ms_w / max(launches_w, 1)
     traffic = GEMM1_TRAFFIC.get((m, n, args.rank, args.nb))
+    alg_bytes = gemm1_algorithmic_bytes(m, n, last_nc)
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFS,
                 "unit": "TF/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4),
-                "traffic": traffic,
+                "traffic": traffic, "algorithmic_bytes": round(alg_bytes),
                 "kernel": "k_gemm1 (trailing update W = T^T V^T C), %d launches, avg %.4f ms"
                           % (launches_w, avg_w)}
     achieved_a = bytes_a / (ms_a / 1000.0) / 1e9 if ms_a > 0 else 0.0

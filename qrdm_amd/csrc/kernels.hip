@@ -37,12 +37,22 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
 // where squaring is exact enough (the role of dnrm2's scaling).
 __device__ double wave_nrm2(const double* __restrict__ x, int n) {
   const int lane = threadIdx.x & 63;
-  double s = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    double v = x[i];
-    s = fma(v, v, s);
+  // four independent loads and partial sums per lane per iteration, so the
+  // loads of one column are in flight together
+  double s0 = 0.0, s1 = 0.0, s2a = 0.0, s3 = 0.0;
+  int i = lane;
+  for (; i + 192 < n; i += 256) {
+    const double v0 = x[i], v1 = x[i + 64], v2 = x[i + 128], v3 = x[i + 192];
+    s0 = fma(v0, v0, s0);
+    s1 = fma(v1, v1, s1);
+    s2a = fma(v2, v2, s2a);
+    s3 = fma(v3, v3, s3);
   }
-  s = wave_sum(s);
+  for (; i < n; i += 64) {
+    const double v = x[i];
+    s0 = fma(v, v, s0);
+  }
+  double s = wave_sum((s0 + s1) + (s2a + s3));
   if (isnan(s)) return s;
   if (s >= 1e-280 && s <= 1e280) return sqrt(s);
   double amax = 0.0;
@@ -82,12 +92,19 @@ __device__ double block_max(double v, double* red) {
 
 // Block-wide two-norm of x[0..n) (scaled fallback as in wave_nrm2).
 __device__ double block_nrm2(const double* __restrict__ x, int n, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double v = x[i];
-    s = fma(v, v, s);
+  const int nt = blockDim.x;
+  double s0 = 0.0, s1 = 0.0;
+  int i = threadIdx.x;
+  for (; i + nt < n; i += 2 * nt) {
+    const double v0 = x[i], v1 = x[i + nt];
+    s0 = fma(v0, v0, s0);
+    s1 = fma(v1, v1, s1);
   }
-  s = block_sum(s, red);
+  for (; i < n; i += nt) {
+    const double v = x[i];
+    s0 = fma(v, v, s0);
+  }
+  double s = block_sum(s0 + s1, red);
   if (isnan(s)) return s;
   if (s >= 1e-280 && s <= 1e280) return sqrt(s);
   double amax = 0.0;
@@ -262,14 +279,20 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
   __shared__ double X[KMAX * (GRAM_ROWS + 1)];
   const int nc = ncols_dev ? *ncols_dev : ncols_max;
   const int rs = r0 + blockIdx.x * GRAM_ROWS;
-  for (int t = threadIdx.x; t < KMAX * GRAM_ROWS; t += blockDim.x) {
-    int c = t / GRAM_ROWS, r = t % GRAM_ROWS;
-    double x = 0.0;
-    if (c < nc && rs + r < r1) {
-      int cc = cols ? cols[c] : c;
-      x = base[(size_t)cc * ld + rs + r];
-    }
-    X[c * (GRAM_ROWS + 1) + r] = x;
+  // 256 threads: 16 elements each, all loads issued before the LDS writes
+  double xv[KMAX * GRAM_ROWS / 256];
+#pragma unroll
+  for (int q = 0; q < KMAX * GRAM_ROWS / 256; q++) {
+    const int t = threadIdx.x + 256 * q;
+    const int c = t / GRAM_ROWS, r = t % GRAM_ROWS;
+    const bool ok = c < nc && rs + r < r1;
+    const int cc = ok ? (cols ? cols[c] : c) : 0;
+    xv[q] = ok ? base[(size_t)cc * ld + rs + r] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < KMAX * GRAM_ROWS / 256; q++) {
+    const int t = threadIdx.x + 256 * q;
+    X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
   }
   __syncthreads();
   const int p0 = (threadIdx.x >> 4) * 4, q0 = (threadIdx.x & 15) * 4;
@@ -368,15 +391,29 @@ __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, d
                                                     const int* __restrict__ perm, int npair) {
   extern __shared__ double buf[];  // npair * PERM_ROWS
   const int rs = blockIdx.x * PERM_ROWS;
-  for (int t = threadIdx.x; t < npair * PERM_ROWS; t += blockDim.x) {
-    int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
-    if (r < m) buf[t] = A[(size_t)perm[2 * p + 1] * lda + r];
+  // npair <= 2 * KMAX: at most 32 elements per thread (256 threads), loaded
+  // together before any store
+  const int tot = npair * PERM_ROWS;
+  for (int q0 = 0; q0 < 2 * KMAX * PERM_ROWS / 256; q0 += 8) {
+    if (q0 * 256 >= tot) break;
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int t = threadIdx.x + 256 * (q0 + q);
+      const int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
+      v[q] = (t < tot && r < m) ? A[(size_t)perm[2 * p + 1] * lda + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int t = threadIdx.x + 256 * (q0 + q);
+      if (t < tot) buf[t] = v[q];
+    }
   }
   double wv = 0.0;
   if (blockIdx.x == 0 && threadIdx.x < npair) wv = work[perm[2 * threadIdx.x + 1]];
   __syncthreads();
-  for (int t = threadIdx.x; t < npair * PERM_ROWS; t += blockDim.x) {
-    int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
+  for (int t = threadIdx.x; t < tot; t += 256) {
+    const int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
     if (r < m) A[(size_t)perm[2 * p] * lda + r] = buf[t];
   }
   if (blockIdx.x == 0 && threadIdx.x < npair) work[perm[2 * threadIdx.x]] = wv;
@@ -538,6 +575,7 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // workgroup writes its slab's partial; the last one of a column tile (atomic
 // ticket) sums the partials in slab order and applies T^T (or T).
 constexpr int G1_CHUNK = 32;
+constexpr int G1_MAX_SPLIT = 8;  // row slabs of the split reduction (Wp holds this many)
 
 __device__ __forceinline__ void load_rows8(const double* __restrict__ col, int r, int rlo, int rhi,
                                            double (&out)[8]) {
@@ -685,13 +723,32 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
   __threadfence();
   // last workgroup of this column tile: sum the partials in slab order into
   // LDS (each element read once, coalesced), then apply op(T)
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) Ts[e & 63][e >> 6] = T[e];
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+  {
+    double tv[KMAX * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) tv[q] = T[tid + 256 * q];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      Ts[e & 63][e >> 6] = tv[q];
+    }
+  }
+  // slab partials summed in slab order; every load of an element is issued
+  // before the sum (slab index clamped, the surplus terms masked)
+  const int ny = gridDim.y;
+#pragma unroll 4
+  for (int q = 0; q < KMAX * KMAX / 256; q++) {
+    const int e = tid + 256 * q;
     const int i = e >> 6, cc = cb + (e & 63);
+    const int ccl = cc < ncol ? cc : ncol - 1;
+    double pv[G1_MAX_SPLIT];
+#pragma unroll
+    for (int y = 0; y < G1_MAX_SPLIT; y++)
+      pv[y] = Wp[((size_t)(y < ny ? y : ny - 1) * KMAX + i) * ldw + ccl];
     double sum = 0.0;
-    if (cc < ncol)
-      for (int y = 0; y < (int)gridDim.y; y++) sum += Wp[((size_t)y * KMAX + i) * ldw + cc];
-    Ws[i][e & 63] = sum;
+#pragma unroll
+    for (int y = 0; y < G1_MAX_SPLIT; y++) sum += (y < ny) ? pv[y] : 0.0;
+    Ws[i][e & 63] = (cc < ncol) ? sum : 0.0;
   }
   __syncthreads();
   const int k0 = 4 * (tid >> 4), q0 = 4 * (tid & 15);
@@ -743,9 +800,19 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
       cval[t][v] = (cc < ncol && r < m) ? col[r] : 0.0;
     }
   }
-  for (int e = tid; e < 64 * KMAX; e += blockDim.x) {
-    const int k = e >> 6, r = e & 63;
-    Vs[r][k] = (rbase + r < m) ? V[(size_t)k * ldv + rbase + r] : 0.0;
+  {
+    double vv[64 * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < 64 * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      const int k = e >> 6, r = e & 63;
+      vv[q] = (rbase + r < m) ? V[(size_t)k * ldv + rbase + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 64 * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      Vs[e & 63][e >> 6] = vv[q];
+    }
   }
   __syncthreads();
   dbl4 acc[4];
@@ -787,6 +854,7 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
   // column tiles
   int want = (1024 + ctiles - 1) / ctiles;
   if (want > nsplit) want = nsplit;
+  if (want > G1_MAX_SPLIT) want = G1_MAX_SPLIT;
   const int base = j & ~(G1_CHUNK - 1);
   const int span = m - base;
   int slab = (span + want - 1) / want;

@@ -28,10 +28,12 @@
 // tau = 2 or tau = 0, non-finite values) sets DevState::flags bit 1 before
 // anything is written, and the caller runs the exact panel instead.
 //
-// The one-workgroup kernels hold a 64 x 64 matrix as 4 x 4 register tiles
-// (thread t: rows 4*(t/16).., columns 4*(t%16)..).  Each elimination step
-// publishes the pivot row or column to LDS, every thread reads what it needs
-// once and applies a masked, branch-free update: one barrier per step.
+// The one-workgroup kernels (k_chol1, k_recon) keep 64 x 64 matrices in LDS
+// and work in 16-column blocks: wave 0 factors a diagonal block with one lane
+// per column, passing pivots and pivot rows between lanes with readlane (no
+// barrier inside the block); all threads then solve the block row or column
+// (one lane per row or column, independent), and the matrix cores update the
+// trailing matrix.  Three barriers per block instead of one per column.
 #include "qrdm_internal.h"
 
 #include <math.h>
@@ -41,160 +43,340 @@ namespace qrdm {
 constexpr int PR_ROWS = 64;  // rows per workgroup of passes 2 and 3
 constexpr double kEpsP = 1.1102230246251565e-16;
 constexpr int LDP = KMAX + 1;  // padded leading dimension of LDS matrices
+constexpr int BLK = 16;        // block width of the one-workgroup factorisations
 
 typedef double dbl4p __attribute__((ext_vector_type(4)));
 
-#define TILE_LOOP                \
-  _Pragma("unroll") for (int a = 0; a < 4; a++) _Pragma("unroll") for (int b = 0; b < 4; b++)
-
-struct Tile {
-  int tid, r0, c0;
-  __device__ Tile() : tid(threadIdx.x), r0(4 * (threadIdx.x >> 4)), c0(4 * (threadIdx.x & 15)) {}
-};
-
-// Publishing a row (column) writes the whole 4-row (4-column) group that
-// holds it: grp[q * KMAX + x] = element (4*(row/4) + q, x), so the register
-// tile is only indexed with compile-time indices (a runtime row index would
-// push the tile to scratch memory).  Readers use grp + (row & 3) * KMAX.
-__device__ __forceinline__ void pub_row(double* grp, const double (&w)[4][4], int row,
-                                        const Tile& t) {
-  if (t.r0 == (row & ~3)) {
-    TILE_LOOP { grp[a * KMAX + t.c0 + b] = w[a][b]; }
-  }
+// Value of x in lane l (l a compile-time or wave-uniform index).
+__device__ __forceinline__ double rdlane(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-__device__ __forceinline__ void pub_col(double* grp, const double (&w)[4][4], int col,
-                                        const Tile& t) {
-  if (t.c0 == (col & ~3)) {
-    TILE_LOOP { grp[b * KMAX + t.r0 + a] = w[a][b]; }
-  }
-}
-
-__device__ __forceinline__ void scale_row(double (&w)[4][4], int row, double s, const Tile& t) {
-  const bool mine = t.r0 == (row & ~3);
+// S[r][c] -= sum_{t < 16} A(r, t) B(t, c) for r, c in [lo, nlim), 16 x 16
+// tiles on the matrix cores (wave w takes tiles w, w + 4, ...).
+// sym: A(r, t) = S[bs + t][r] and only tiles on or above the diagonal (the
+// symmetric Cholesky update); otherwise A(r, t) = S[r][bs + t].
+// B(t, c) = S[bs + t][c].  The tiles written are disjoint from the rows
+// bs..bs+15 and columns bs..bs+15 read.
+__device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, int nlim,
+                                              bool sym) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, li = lane & 15;
+  const int nt = (KMAX - lo) >> 4;
+  int e = 0;
+  for (int tr = 0; tr < nt; tr++) {
+    for (int tc = sym ? tr : 0; tc < nt; tc++, e++) {
+      const int r0 = lo + 16 * tr, c0 = lo + 16 * tc;
+      if ((e & 3) != wave || r0 >= nlim || c0 >= nlim) continue;
+      dbl4p acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const double f = (mine && (row & 3) == a) ? s : 1.0;
+      for (int k0 = 0; k0 < BLK; k0 += 4) {
+        const double a = sym ? S[bs + k0 + g][r0 + li] : S[r0 + li][bs + k0 + g];
+        const double b = S[bs + k0 + g][c0 + li];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
 #pragma unroll
-    for (int b = 0; b < 4; b++) w[a][b] *= f;
+      for (int v = 0; v < 4; v++) S[r0 + g + 4 * v][c0 + li] -= acc[v];
+    }
   }
 }
 
-__device__ __forceinline__ void tile_load(double (&w)[4][4], const double (*S)[LDP], const Tile& t) {
-  TILE_LOOP { w[a][b] = S[t.r0 + a][t.c0 + b]; }
-}
-
-__device__ __forceinline__ void tile_store(double (*S)[LDP], const double (&w)[4][4], const Tile& t) {
-  TILE_LOOP { S[t.r0 + a][t.c0 + b] = w[a][b]; }
-}
-
-__device__ __forceinline__ void tile_identity(double (&w)[4][4], int n, const Tile& t) {
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    w[a][b] = (ra == cb && ra < n) ? 1.0 : 0.0;
-  }
-}
-
-// w = X Y for 64 x 64 LDS operands (zeros outside their triangles do the masking)
-__device__ __forceinline__ void tile_gemm(double (&w)[4][4], const double (*X)[LDP],
-                                          const double (*Y)[LDP], const Tile& t) {
-  TILE_LOOP { w[a][b] = 0.0; }
-  for (int q = 0; q < KMAX; q++) {
-    double xv[4], yv[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) xv[a] = X[t.r0 + a][q];
-#pragma unroll
-    for (int b = 0; b < 4; b++) yv[b] = Y[q][t.c0 + b];
-    TILE_LOOP { w[a][b] = fma(xv[a], yv[b], w[a][b]); }
-  }
-}
-
-// In-place upper Cholesky of the n x n tiled matrix (upper part used).
-// Leaves the unscaled rows in w and the pivots d_i in dpiv; row i of the
-// factor is w(i, :) / sqrt(d_i).  stopfn(i, d) returns 0 to continue; the
-// return value is the number of columns factored, *why the stop code.
+// In-place upper Cholesky (R^T R = S, upper part of S read) of the leading
+// nlim x nlim block; 256 threads.  stopfn(i, d) is called with each pivot d
+// before it is used and returns 0 to continue, else a stop code: the
+// factorisation then ends before column i.  Returns the number of columns
+// factored (rows < that of S then hold R) and the stop code in *why.
+// rinv[i] = 1 / R(i, i).
 template <class StopFn>
-__device__ __forceinline__ int tile_chol(double (&w)[4][4], int n, double* rb, double* dpiv, StopFn stopfn,
-                         int* why, const Tile& t) {
-  pub_row(rb, w, 0, t);
-  __syncthreads();
-  for (int i = 0; i < n; i++) {
-    const double* buf = rb + (i & 1) * 4 * KMAX + (i & 3) * KMAX;
-    const double d = buf[i];
-    const int st = stopfn(i, d);
-    if (st) {
-      *why = st;
-      return i;
-    }
-    if (t.tid == 0) dpiv[i] = d;
-    const double rd = 1.0 / d;
-    double rv[4], cv[4];
+__device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn stopfn, double* rinv,
+                           int* s_done, int* s_why, int* why) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int bs = 0; bs < nlim; bs += BLK) {
+    if (wave == 0) {
+      // lane c holds column bs + c of the block
+      const int c = lane & 15;
+      double col[BLK];
 #pragma unroll
-    for (int a = 0; a < 4; a++) rv[a] = buf[t.r0 + a];
+      for (int r = 0; r < BLK; r++) col[r] = S[bs + r][bs + c];
+      bool act = true;
+      int done = BLK, code = 0;
 #pragma unroll
-    for (int b = 0; b < 4; b++) cv[b] = buf[t.c0 + b];
-    TILE_LOOP {
-      const int ra = t.r0 + a, cb = t.c0 + b;
-      const double f = (ra > i && cb >= ra) ? rv[a] * rd : 0.0;
-      w[a][b] = fma(-f, cv[b], w[a][b]);
+      for (int i = 0; i < BLK; i++) {
+        if (act) {
+          const int gi = bs + i;
+          if (gi >= nlim) {
+            act = false;
+            done = i;
+          } else {
+            const double d = rdlane(col[i], i);
+            const int stc = stopfn(gi, d);
+            if (stc) {
+              act = false;
+              done = i;
+              code = stc;
+            } else {
+              const double rs = 1.0 / sqrt(d);
+              const double ri = col[i] * rs;  // R(i, c)
+              col[i] = ri;
+              if (lane == 0) rinv[gi] = rs;
+#pragma unroll
+              for (int r = i + 1; r < BLK; r++) col[r] = fma(-rdlane(ri, r), ri, col[r]);
+            }
+          }
+        }
+      }
+      if (lane < BLK) {
+#pragma unroll
+        for (int r = 0; r < BLK; r++) S[bs + r][bs + c] = (r < done && c < r) ? 0.0 : col[r];
+      }
+      if (lane == 0) {
+        *s_done = done;
+        *s_why = code;
+      }
     }
-    if (i + 1 < n) pub_row(rb + ((i + 1) & 1) * 4 * KMAX, w, i + 1, t);
+    __syncthreads();
+    const int done = *s_done, code = *s_why;
+    if (done < BLK || code) {
+      *why = code;
+      return bs + done;
+    }
+    const int lo = bs + BLK;
+    if (lo >= nlim) break;
+    // block row: R(bs.., c) = R_bb^-T S(bs.., c), one thread per column
+    for (int c = lo + tid; c < nlim; c += blockDim.x) {
+      double x[BLK];
+#pragma unroll
+      for (int i = 0; i < BLK; i++) x[i] = S[bs + i][c];
+#pragma unroll
+      for (int i = 0; i < BLK; i++) {
+        double sacc = x[i];
+#pragma unroll
+        for (int l = 0; l < i; l++) sacc = fma(-S[bs + l][bs + i], x[l], sacc);
+        x[i] = sacc * rinv[bs + i];
+      }
+#pragma unroll
+      for (int i = 0; i < BLK; i++) S[bs + i][c] = x[i];
+    }
+    __syncthreads();
+    mfma_trailing(S, bs, lo, nlim, true);
     __syncthreads();
   }
   *why = 0;
-  return n;
+  return nlim;
 }
 
-// w = R^-1 for the n x n upper triangular R in LDS (R zero outside 0..n-1):
-// rows of R X = I from the bottom up, each final row scaled and published,
-// then removed from the rows above it.
-__device__ __forceinline__ void tile_triinv_upper(double (&w)[4][4], const double (*R)[LDP], int n, double* rb,
-                                  const Tile& t) {
-  tile_identity(w, n, t);
-  if (n <= 0) return;
-  scale_row(w, n - 1, 1.0 / R[n - 1][n - 1], t);
-  pub_row(rb + ((n - 1) & 1) * 4 * KMAX, w, n - 1, t);
-  __syncthreads();
-  for (int i = n - 1; i >= 0; i--) {
-    const double* buf = rb + (i & 1) * 4 * KMAX + (i & 3) * KMAX;
-    double rv[4], xv[4];
+// In-place LU without pivoting of the Householder reconstruction's running
+// matrix W (the unsigned Q11, see k_recon) on the leading nlim x nlim block;
+// 256 threads.  Column i is eliminated with the pivot d_i a_i - 1 of
+// Q11 D - I, d_i = -sign(a_i) (dlarfg's beta = -sign(alpha) ||.||), a_i the
+// running diagonal: multipliers L(r, i) = d_i W(r, i) / (-1 - |a_i|); the
+// identity part never enters the updates, so W keeps the unsigned U rows.
+// Before column i the DM truncation test runs on ||x_i|| = R(i,i)
+// sqrt(1 - a_i^2) (stop code 1 for i >= 1 below thres) and degenerate
+// pivots stop with code 2.  Returns the number of columns eliminated;
+// dsg[i] = d_i, tl[i] = 1 + |a_i| (= tau_i), scv[i] = the multiplier scale.
+__device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, const double (*SR)[LDP],
+                                 double thres, double* dsg, double* tl, double* scv,
+                                 int* s_done, int* s_why, int* why) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int bs = 0; bs < nlim; bs += BLK) {
+    if (wave == 0) {
+      const int c = lane & 15;
+      double col[BLK];
 #pragma unroll
-    for (int a = 0; a < 4; a++) rv[a] = R[t.r0 + a][i];
+      for (int r = 0; r < BLK; r++) col[r] = W[bs + r][bs + c];
+      bool act = true;
+      int done = BLK, code = 0;
 #pragma unroll
-    for (int b = 0; b < 4; b++) xv[b] = buf[t.c0 + b];
-    TILE_LOOP {
-      const int ra = t.r0 + a;
-      const double f = (ra < i) ? rv[a] : 0.0;
-      w[a][b] = fma(-f, xv[b], w[a][b]);
+      for (int i = 0; i < BLK; i++) {
+        if (act) {
+          const int gi = bs + i;
+          if (gi >= nlim) {
+            act = false;
+            done = i;
+          } else {
+            const double al = rdlane(col[i], i);
+            const double aa = fabs(al);
+            const double om = 1.0 - aa;  // 2 - tau
+            const double xnorm = SR[gi][gi] * sqrt(fmax(om, 0.0) * (1.0 + aa));
+            int stc = 0;
+            if (!(al == al) || !(xnorm == xnorm)) stc = 2;
+            else if (gi >= 1 && xnorm < thres) stc = 1;
+            else if (om < 1e-6) stc = 2;  // tau near 2: dlarfg's tau = 0 branch or lost digits
+            if (stc) {
+              act = false;
+              done = i;
+              code = stc;
+            } else {
+              const double di = (al >= 0.0) ? -1.0 : 1.0;
+              const double sc = di / (-1.0 - aa);
+              if (lane == 0) {
+                dsg[gi] = di;
+                tl[gi] = 1.0 + aa;
+                scv[gi] = sc;
+              }
+              const double wic = col[i];  // W(i, c)
+#pragma unroll
+              for (int r = i + 1; r < BLK; r++) {
+                const double lv = rdlane(col[r], i) * sc;  // L(r, i)
+                const double upd = fma(-lv, wic, col[r]);
+                col[r] = (c > i) ? upd : ((c == i) ? lv : col[r]);
+              }
+            }
+          }
+        }
+      }
+      if (lane < BLK) {
+#pragma unroll
+        for (int r = 0; r < BLK; r++) W[bs + r][bs + c] = col[r];
+      }
+      if (lane == 0) {
+        *s_done = done;
+        *s_why = code;
+      }
     }
-    if (i >= 1) {
-      scale_row(w, i - 1, 1.0 / R[i - 1][i - 1], t);
-      pub_row(rb + ((i - 1) & 1) * 4 * KMAX, w, i - 1, t);
+    __syncthreads();
+    const int done = *s_done, code = *s_why;
+    if (done < BLK || code) {
+      *why = code;
+      return bs + done;
+    }
+    const int lo = bs + BLK;
+    if (lo >= nlim) break;
+    if (tid < 64) {
+      // block column below: L(r, bs..) by forward substitution with U_bb
+      const int r = lo + tid;
+      if (r < nlim) {
+        double x[BLK];
+#pragma unroll
+        for (int i = 0; i < BLK; i++) x[i] = W[r][bs + i];
+#pragma unroll
+        for (int i = 0; i < BLK; i++) {
+          double sacc = x[i];
+#pragma unroll
+          for (int l = 0; l < i; l++) sacc = fma(-x[l], W[bs + l][bs + i], sacc);
+          x[i] = sacc * scv[bs + i];
+        }
+#pragma unroll
+        for (int i = 0; i < BLK; i++) W[r][bs + i] = x[i];
+      }
+    } else if (tid < 128) {
+      // block row to the right: U(bs.., c) = L_bb^-1 W(bs.., c)
+      const int c = lo + tid - 64;
+      if (c < nlim) {
+        double y[BLK];
+#pragma unroll
+        for (int i = 0; i < BLK; i++) y[i] = W[bs + i][c];
+#pragma unroll
+        for (int i = 0; i < BLK; i++) {
+          double sacc = y[i];
+#pragma unroll
+          for (int l = 0; l < i; l++) sacc = fma(-W[bs + i][bs + l], y[l], sacc);
+          y[i] = sacc;
+        }
+#pragma unroll
+        for (int i = 0; i < BLK; i++) W[bs + i][c] = y[i];
+      }
+    }
+    __syncthreads();
+    mfma_trailing(W, bs, lo, nlim, false);
+    __syncthreads();
+  }
+  *why = 0;
+  return nlim;
+}
+
+// X = R^-1 for a 64 x 64 upper triangular R with a non-zero diagonal
+// (R(i, l) = Rin[l][i] when trans, else Rin[i][l]); 256 threads.  The four
+// 16 x 16 diagonal blocks are inverted by the four waves (one lane per
+// column, back substitution), then the blocks above by distance d = 1..3:
+// X_ij = -X_ii sum_{k=i+1..j} R_ik X_kj.  Ends with a barrier.
+__device__ __forceinline__ void tri_inv_upper(const double (*Rin)[LDP], bool trans, double (*X)[LDP],
+                              double* rinv, double (*Y)[BLK][BLK + 1]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  auto R = [&](int i, int l) -> double { return trans ? Rin[l][i] : Rin[i][l]; };
+  if (tid < KMAX) rinv[tid] = 1.0 / R(tid, tid);
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    if ((r >> 4) > (c >> 4)) X[r][c] = 0.0;
+  }
+  __syncthreads();
+  {
+    const int base = BLK * wave, lc = lane & 15;
+    double x[BLK];
+#pragma unroll
+    for (int i = BLK - 1; i >= 0; i--) {
+      double sacc = (i == lc) ? 1.0 : 0.0;
+#pragma unroll
+      for (int l = i + 1; l < BLK; l++) sacc = fma(-R(base + i, base + l), x[l], sacc);
+      x[i] = sacc * rinv[base + i];
+    }
+    if (lane < BLK) {
+#pragma unroll
+      for (int i = 0; i < BLK; i++) X[base + i][base + lc] = x[i];
+    }
+  }
+  __syncthreads();
+  const int r = tid >> 4, c = tid & 15;
+  for (int d = 1; d < 4; d++) {
+    for (int bi = 0; bi + d < 4; bi++) {
+      const int bj = bi + d;
+      double sacc = 0.0;
+      for (int kb = bi + 1; kb <= bj; kb++) {
+#pragma unroll
+        for (int t = 0; t < BLK; t++)
+          sacc = fma(R(BLK * bi + r, BLK * kb + t), X[BLK * kb + t][BLK * bj + c], sacc);
+      }
+      Y[bi][r][c] = sacc;
+    }
+    __syncthreads();
+    for (int bi = 0; bi + d < 4; bi++) {
+      const int bj = bi + d;
+      double sacc = 0.0;
+#pragma unroll
+      for (int t = 0; t < BLK; t++) sacc = fma(X[BLK * bi + r][BLK * bi + t], Y[bi][t][c], sacc);
+      X[BLK * bi + r][BLK * bj + c] = -sacc;
     }
     __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
+// R1 = chol(G) of the panel's Gram (gathered from the candidate Gram through
+// the rank map), stopped at the first pivot that proves the DM truncation
+// fires there; R1^-1.  Identity padding outside fjb.
 __global__ __launch_bounds__(256) void k_chol1(const double* __restrict__ Gc,
                                                const int* __restrict__ rank, int fjb,
                                                double tau_dm, double* R1, double* R1inv,
                                                DevState* st) {
-  __shared__ double Rs[KMAX][LDP];
-  __shared__ double rb[8 * KMAX];
-  __shared__ double dpiv[KMAX], diag[KMAX];
+  __shared__ double S[KMAX][LDP];
+  __shared__ double X[KMAX][LDP];
+  __shared__ double Y[3][BLK][BLK + 1];
+  __shared__ double rinv[KMAX], diag[KMAX];
   __shared__ int srk[KMAX];
-  const Tile t;
-  if (t.tid < KMAX) {
-    const int rk = t.tid < fjb ? rank[t.tid] : 0;
-    srk[t.tid] = rk;
-    diag[t.tid] = t.tid < fjb ? Gc[rk * (KMAX + 1)] : 0.0;
+  __shared__ int s_done, s_why;
+  const int tid = threadIdx.x;
+  if (tid < KMAX) {
+    const int rk = tid < fjb ? rank[tid] : 0;
+    srk[tid] = rk;
+    diag[tid] = tid < fjb ? Gc[rk * (KMAX + 1)] : 0.0;
   }
   __syncthreads();
-  double w[4][4];
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    w[a][b] = (ra < fjb && cb < fjb) ? Gc[srk[ra] + srk[cb] * KMAX] : 0.0;
+  {
+    double gv[KMAX * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      gv[q] = (r < fjb && c < fjb) ? Gc[srk[r] + srk[c] * KMAX] : (r == c ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      S[e >> 6][e & 63] = gv[q];
+    }
   }
   double maxd = 0.0;
   for (int q = 0; q < KMAX; q++) maxd = fmax(maxd, diag[q]);
@@ -203,31 +385,36 @@ __global__ __launch_bounds__(256) void k_chol1(const double* __restrict__ Gc,
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * sqrt(g00) : 5e-14;
   const double thr2 = 0.25 * thres * thres;
   const bool scale_ok = g00 > 1e-280 && g00 < 1e280 && maxd < 1e280;
+  __syncthreads();
   int why = 0;
-  const int k1 = tile_chol(
-      w, fjb, rb, dpiv,
+  const int k1 = chol_blocked(
+      S, fjb,
       [&](int i, double d) -> int {
         if (!scale_ok) return 2;
         if (i >= 1 && d < thr2 && thr2 > tolp) return 1;  // truncation at <= i is certain
         if (!(d > tolp)) return 2;
         return 0;
       },
-      &why, t);
+      rinv, &s_done, &s_why, &why);
   if (why == 2) {
-    if (t.tid == 0) st->flags |= 2;
+    if (tid == 0) st->flags |= 2;
     return;
   }
-  __syncthreads();
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    const double v = (ra <= cb && cb < k1) ? w[a][b] / sqrt(dpiv[ra]) : 0.0;
-    Rs[ra][cb] = v;
-    R1[ra + cb * KMAX] = v;
+  // R1 (zero outside k1) out; identity-padded R1 in S for the inverse
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    const bool in = r < k1 && c < k1;
+    const double v = (in && c >= r) ? S[r][c] : 0.0;
+    R1[r + c * KMAX] = v;
+    S[r][c] = in ? v : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  tile_triinv_upper(w, Rs, k1, rb, t);
-  TILE_LOOP { R1inv[(t.r0 + a) + (t.c0 + b) * KMAX] = w[a][b]; }
-  if (t.tid == 0) st->k1 = k1;
+  tri_inv_upper(S, false, X, rinv, Y);
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    R1inv[r + c * KMAX] = (r < k1 && c < k1) ? X[r][c] : 0.0;
+  }
+  if (tid == 0) st->k1 = k1;
 }
 
 // ---------------------------------------------------------------------------
@@ -259,6 +446,37 @@ __device__ __forceinline__ void mfma_store(double (*S)[LDP], const dbl4p (&acc)[
     for (int v = 0; v < 4; v++) S[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] = acc[u][v];
 }
 
+// Ps[r][c] = A(rs + r, j + c) for c < ncol and rows < m, else 0 (a 64-row
+// slab of the panel); 256 threads, all loads in flight before the LDS writes.
+__device__ __forceinline__ void load_slab(double (*Ps)[LDP], const double* __restrict__ A,
+                                          int lda, int j, int rs, int m, int ncol) {
+  const int tid = threadIdx.x;
+  double v[KMAX * PR_ROWS / 256];
+#pragma unroll
+  for (int q = 0; q < KMAX * PR_ROWS / 256; q++) {
+    const int e = tid + 256 * q, c = e / PR_ROWS, r = e % PR_ROWS;
+    v[q] = (c < ncol && rs + r < m) ? A[(size_t)(j + c) * lda + rs + r] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < KMAX * PR_ROWS / 256; q++) {
+    const int e = tid + 256 * q;
+    Ps[e % PR_ROWS][e / PR_ROWS] = v[q];
+  }
+}
+
+// S[r][c] = M[r + c * KMAX] for a column-major 64 x 64 matrix in memory.
+__device__ __forceinline__ void load_mat(double (*S)[LDP], const double* __restrict__ M) {
+  const int tid = threadIdx.x;
+  double v[KMAX * KMAX / 256];
+#pragma unroll
+  for (int q = 0; q < KMAX * KMAX / 256; q++) v[q] = M[tid + 256 * q];
+#pragma unroll
+  for (int q = 0; q < KMAX * KMAX / 256; q++) {
+    const int e = tid + 256 * q;
+    S[e & 63][e >> 6] = v[q];
+  }
+}
+
 // Pass 2: Q = P R1^-1 on a 64-row slab, partial Gram Q^T Q.
 __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int lda, int j,
                                                int m, const double* __restrict__ R1inv,
@@ -269,11 +487,8 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
   const int k1 = st->k1;
   const int tid = threadIdx.x;
   const int rs = j + blockIdx.x * PR_ROWS;
-  for (int e = tid; e < KMAX * PR_ROWS; e += blockDim.x) {
-    const int c = e / PR_ROWS, r = e % PR_ROWS;
-    Ps[r][c] = (c < k1 && rs + r < m) ? A[(size_t)(j + c) * lda + rs + r] : 0.0;
-  }
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) Ri[e & 63][e >> 6] = R1inv[e];
+  load_slab(Ps, A, lda, j, rs, m, k1);
+  load_mat(Ri, R1inv);
   __syncthreads();
   dbl4p acc[4];
   mfma_64(acc, Ps, false, Ri);
@@ -299,141 +514,120 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
                                                int lda, int j, int fjb, double tau_dm,
                                                double* tau, double* T_out, double* M_out,
                                                DevState* st) {
-  __shared__ double SA[KMAX][LDP];  // R2, then R^-1
-  __shared__ double SB[KMAX][LDP];  // R1, then R
-  __shared__ double SC[KMAX][LDP];  // P_top, then L
-  __shared__ double SD[KMAX][LDP];  // U, then U^-1
-  __shared__ double rb[8 * KMAX], cl[8 * KMAX];
-  __shared__ double dsg[KMAX], tl[KMAX], dpiv[KMAX];
+  __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
+  __shared__ double SA[KMAX][LDP];  // R^-1
+  __shared__ double SW[KMAX][LDP];  // P_top, then Q11 and its LU, then L^-T
+  __shared__ double SX[KMAX][LDP];  // G2 and R2, then U
+  __shared__ double Y[3][BLK][BLK + 1];
+  __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
+  __shared__ int s_done, s_why;
   if (st->flags & 2) return;
-  const Tile t;
+  const int tid = threadIdx.x;
   const int k1 = st->k1;
-  double w[4][4];
-  // 1. R2 = chol(G2); G2 is close to I after the first pass
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    w[a][b] = (ra < k1 && cb < k1) ? G2[ra + cb * KMAX] : 0.0;
+  {
+    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256], gw[KMAX * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      // e walks rows fastest: coalesced column-major reads
+      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+      const bool in = r < k1 && c < k1;
+      gx[q] = in ? G2[r + c * KMAX] : (r == c ? 1.0 : 0.0);
+      gr[q] = R1[r + c * KMAX];
+      gw[q] = in ? A[(size_t)(j + c) * lda + j + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+      SX[r][c] = gx[q];
+      SR[r][c] = gr[q];
+      SW[r][c] = gw[q];
+    }
   }
+  __syncthreads();
+  // 1. R2 = chol(G2); G2 is close to I after the first pass
   int why = 0;
-  tile_chol(
-      w, k1, rb, dpiv, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; },
-      &why, t);
+  chol_blocked(
+      SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv, &s_done,
+      &s_why, &why);
   if (why) {
-    if (t.tid == 0) st->flags |= 2;
+    if (tid == 0) st->flags |= 2;
     return;
   }
-  __syncthreads();
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    SA[ra][cb] = (ra <= cb && cb < k1) ? w[a][b] / sqrt(dpiv[ra]) : 0.0;
-    SB[ra][cb] = R1[ra + cb * KMAX];
-    SC[ra][cb] = (ra < k1 && cb < k1) ? A[(size_t)(j + cb) * lda + j + ra] : 0.0;
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
   }
   __syncthreads();
-  // 2. R = R2 R1
-  tile_gemm(w, SA, SB, t);
+  // 2. R = R2 R1 (identity-padded for the inverse)
+  dbl4p acc[4];
+  mfma_64(acc, SX, false, SR);
   __syncthreads();
-  tile_store(SB, w, t);
+  mfma_store(SR, acc);
+  __syncthreads();
+  if (tid >= k1 && tid < KMAX) SR[tid][tid] = 1.0;
   __syncthreads();
   // 3. R^-1 -> SA
-  tile_triinv_upper(w, SB, k1, rb, t);
-  tile_store(SA, w, t);
+  tri_inv_upper(SR, false, SA, rinv, Y);
+  // 4. Q11 = P_top R^-1 -> SW
+  mfma_64(acc, SW, false, SA);
   __syncthreads();
-  // 4. Q11 = P_top R^-1
-  tile_gemm(w, SC, SA, t);
-  // 5. LU of Q11 D - I (unsigned running matrix), dlarfg's signs, truncation
-  const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SB[0][0] : 5e-14;
-  pub_row(rb, w, 0, t);
-  pub_col(cl, w, 0, t);
+  mfma_store(SW, acc);
   __syncthreads();
-  int k = k1;
-  for (int i = 0; i < k1; i++) {
-    const double* rowb = rb + (i & 1) * 4 * KMAX + (i & 3) * KMAX;
-    const double* colb = cl + (i & 1) * 4 * KMAX + (i & 3) * KMAX;
-    const double al = rowb[i];
-    const double aa = fabs(al);
-    const double om = 1.0 - aa;  // 2 - tau
-    const double xnorm = SB[i][i] * sqrt(fmax(om, 0.0) * (1.0 + aa));
-    int stop = 0;
-    if (!(al == al) || !(xnorm == xnorm)) stop = 2;
-    else if (i >= 1 && xnorm < thres) stop = 1;
-    else if (om < 1e-6) stop = 2;  // tau near 2: dlarfg's tau = 0 branch or lost digits
-    if (stop == 1) {
-      k = i;
-      break;
-    }
-    if (stop == 2) {
-      if (t.tid == 0) st->flags |= 2;
-      return;
-    }
-    const double di = (al >= 0.0) ? -1.0 : 1.0;  // beta = -d_sign(norm, alpha)
-    const double sc = di / (-1.0 - aa);           // L column = d_i * column / pivot
-    if (t.tid == 0) {
-      dsg[i] = di;
-      tl[i] = 1.0 + aa;
-    }
-    double rv[4], cv[4];
+  // 5. LU of Q11 D - I with dlarfg's signs and the truncation test
+  const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SR[0][0] : 5e-14;
+  const int k = lu_signed_blocked(SW, k1, SR, thres, dsg, tl, scv, &s_done, &s_why, &why);
+  if (why == 2) {
+    if (tid == 0) st->flags |= 2;
+    return;
+  }
+  // 6. top block (R rows scaled by the signs, V1 below), tau; U -> SX
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    const bool in = r < k && c < k;
+    const double w = SW[r][c];
+    if (in) A[(size_t)(j + c) * lda + j + r] = (r <= c) ? dsg[r] * SR[r][c] : w;
+    SX[r][c] = in ? (r < c ? dsg[c] * w : (r == c ? -tl[c] : 0.0)) : (r == c ? 1.0 : 0.0);
+  }
+  if (tid < k) tau[tid] = tl[tid];
+  __syncthreads();
+  // L^T (unit upper) -> SR
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    SR[r][c] = (r < c && c < k) ? SW[c][r] : (r == c ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  // 7. L^-T = (L^T)^-1 -> SW;  T = -U L^-T  (T V1^T = -U)
+  tri_inv_upper(SR, false, SW, rinv, Y);
+  mfma_64(acc, SX, false, SW);
+  {
+    const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int a = 0; a < 4; a++) rv[a] = colb[t.r0 + a] * sc;
+    for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) cv[b] = rowb[t.c0 + b];
-    TILE_LOOP {
-      const int ra = t.r0 + a, cb = t.c0 + b;
-      const double upd = fma(-rv[a], cv[b], w[a][b]);
-      w[a][b] = (ra > i) ? ((cb == i) ? rv[a] : ((cb > i) ? upd : w[a][b])) : w[a][b];
-    }
-    if (i + 1 < k1) {
-      pub_row(rb + ((i + 1) & 1) * 4 * KMAX, w, i + 1, t);
-      pub_col(cl + ((i + 1) & 1) * 4 * KMAX, w, i + 1, t);
-    }
-    __syncthreads();
+      for (int v = 0; v < 4; v++) {
+        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
+        T_out[r + c * KMAX] = (r < k && c < k) ? -acc[u][v] : 0.0;
+      }
+  }
+  // 8. U^-1 -> SR, rows scaled by D;  M = R^-1 D U^-1
+  tri_inv_upper(SX, false, SR, rinv, Y);
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    SR[r][c] *= (r < k) ? dsg[r] : 0.0;
   }
   __syncthreads();
-  // 6. top block (R rows scaled by the signs, V1 below), tau; L -> SC; U -> SD
-  double u[4][4];
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    const bool in = ra < k && cb < k;
-    if (in) A[(size_t)(j + cb) * lda + j + ra] = (ra <= cb) ? dsg[ra] * SB[ra][cb] : w[a][b];
-    SC[ra][cb] = (in && ra > cb) ? w[a][b] : 0.0;
-    u[a][b] = in ? (ra < cb ? dsg[cb] * w[a][b] : (ra == cb ? -tl[cb] : 0.0)) : 0.0;
-    SD[ra][cb] = u[a][b];
-  }
-  if (t.tid < k) tau[t.tid] = tl[t.tid];
-  // 7. T V1^T = -U, right-looking over columns (V1 unit lower)
-  TILE_LOOP { w[a][b] = -u[a][b]; }
-  pub_col(cl, w, 0, t);
-  __syncthreads();
-  for (int c = 0; c < k; c++) {
-    const double* buf = cl + (c & 1) * 4 * KMAX + (c & 3) * KMAX;
-    double xv[4], lv[4];
+  mfma_64(acc, SA, false, SR);
+  {
+    const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
-    for (int a = 0; a < 4; a++) xv[a] = buf[t.r0 + a];
+    for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int b = 0; b < 4; b++) lv[b] = (t.c0 + b > c) ? SC[t.c0 + b][c] : 0.0;
-    TILE_LOOP { w[a][b] = fma(-xv[a], lv[b], w[a][b]); }
-    if (c + 1 < k) pub_col(cl + ((c + 1) & 1) * 4 * KMAX, w, c + 1, t);
-    __syncthreads();
+      for (int v = 0; v < 4; v++) {
+        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
+        M_out[r + c * KMAX] = (r < k && c < k) ? acc[u][v] : 0.0;
+      }
   }
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    T_out[ra + cb * KMAX] = (ra < k && cb < k) ? w[a][b] : 0.0;
-  }
-  // 8. U^-1 -> SD
-  tile_triinv_upper(w, SD, k, rb, t);
-  __syncthreads();
-  tile_store(SD, w, t);
-  // 9. M = R^-1 D U^-1 (D folded into the rows of U^-1)
-  if (t.tid < KMAX) dpiv[t.tid] = t.tid < k ? dsg[t.tid] : 0.0;
-  __syncthreads();
-  TILE_LOOP { SD[t.r0 + a][t.c0 + b] *= dpiv[t.r0 + a]; }
-  __syncthreads();
-  tile_gemm(w, SA, SD, t);
-  TILE_LOOP {
-    const int ra = t.r0 + a, cb = t.c0 + b;
-    M_out[ra + cb * KMAX] = (ra < k && cb < k) ? w[a][b] : 0.0;
-  }
-  if (t.tid == 0) st->fjb_cmp = k;
+  if (tid == 0) st->fjb_cmp = k;
 }
 
 // ---------------------------------------------------------------------------
@@ -449,11 +643,8 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int j, int m,
   const int k = st->fjb_cmp;
   const int tid = threadIdx.x;
   const int rs = j + blockIdx.x * PR_ROWS;
-  for (int e = tid; e < KMAX * PR_ROWS; e += blockDim.x) {
-    const int c = e / PR_ROWS, r = e % PR_ROWS;
-    Ps[r][c] = (c < k && rs + r < m) ? A[(size_t)(j + c) * lda + rs + r] : 0.0;
-  }
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) Ms[e & 63][e >> 6] = M[e];
+  load_slab(Ps, A, lda, j, rs, m, k);
+  load_mat(Ms, M);
   __syncthreads();
   dbl4p acc[4];
   mfma_64(acc, Ps, false, Ms);
