@@ -125,6 +125,11 @@ int qrdm_permute_walk(int ncols, const int *sel, int nsel, int *swaps);
  * reference (it has no timing facility).
  */
 void qrdm_profile_enable(int on);
+/* Classes timed while enabled: bit c selects class c (default: all).  Each
+ * timed region costs two event records on the stream (a few microseconds of
+ * idle device time each), so a measurement of the whole factorisation should
+ * time only the classes it reports. */
+void qrdm_profile_select(unsigned mask);
 void qrdm_profile_reset(void);
 int qrdm_profile_read(double *out, int nclass);
 

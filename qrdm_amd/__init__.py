@@ -28,7 +28,8 @@ class Step(ctypes.Structure):
 # Every entry point include/qrdm_amd.h declares.
 EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrf_dev",
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
-           "qrdm_profile_enable", "qrdm_profile_reset", "qrdm_profile_read",
+           "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
+           "qrdm_profile_read",
            "qrdm_last_error", "qrdm_version")
 
 PROFILE_CLASSES = ("larfb_w", "norm_update", "select", "panel", "permute", "init_norms",
@@ -58,6 +59,7 @@ def lib():
                                   ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.qrdm_permute_walk.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip]
         L.qrdm_profile_enable.argtypes = [ctypes.c_int]
+        L.qrdm_profile_select.argtypes = [ctypes.c_uint]
         L.qrdm_profile_read.argtypes = [_dp, ctypes.c_int]
         L.qrdm_profile_read.restype = ctypes.c_int
         L.qrdm_last_error.restype = ctypes.c_char_p
@@ -105,11 +107,19 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
     return L.qrdm_dgeqrdm_dev(*args), None
 
 
-def profile(enable=None, reset=False):
+def profile(enable=None, reset=False, classes=None):
     """Reads (and optionally resets / toggles) the library's kernel-time
-    accounting: {class: (ms, launches, work)}."""
+    accounting: {class: (ms, launches, work)}.  classes: names from
+    PROFILE_CLASSES to time while enabled (None: all)."""
     import numpy as np
     L = lib()
+    if classes is not None:
+        mask = 0
+        for c in classes:
+            mask |= 1 << PROFILE_CLASSES.index(c)
+        L.qrdm_profile_select(ctypes.c_uint(mask))
+    elif enable is not None:
+        L.qrdm_profile_select(ctypes.c_uint(0xffffffff))
     if enable is not None:
         L.qrdm_profile_enable(1 if enable else 0)
     out = np.zeros(3 * len(PROFILE_CLASSES))

@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,6 +29,7 @@ thread_local std::string g_err;
 constexpr double kEps = 1.1102230246251565e-16;  // LAPACK dlamch('E') = 2^-53
 // Panels with at least this many rows take the Gram-based path.
 constexpr int kFastPanelRows = 128;
+constexpr int kBatch = 8;  // DM steps queued between synchronisations
 // Layout of the per-step upload: up to 2*KMAX (dst, src) pairs, then KMAX
 // candidate ranks of the panel columns.
 constexpr int kRankOff = 4 * KMAX;
@@ -62,6 +64,7 @@ enum { PC_LARFB = 0, PC_NORM = 1, PC_SELECT = 2, PC_PANEL = 3, PC_PERM = 4, PC_I
        PC_APPLY = 6 };
 struct Profiler {
   bool on = false;
+  unsigned mask = 0xffffffffu;  // bit c: class c is timed
   double ms[kNumClasses] = {}, launches[kNumClasses] = {}, work[kNumClasses] = {};
   std::vector<hipEvent_t> pool;
   size_t used = 0;
@@ -81,7 +84,7 @@ struct Profiler {
   }
   // begin/end bracket one region; work is its algorithmic flops or bytes
   int begin(int cls, hipStream_t s) {
-    if (!on) return -1;
+    if (!on || !((mask >> cls) & 1u)) return -1;
     Rec r{cls, ev(), ev(), 0.0};
     HIPCHK(hipEventRecord(r.a, s));
     open.push_back(r);
@@ -133,6 +136,8 @@ struct Buffers {
   double* Wp;  // nsplit * 64 * n
   double* W;   // 64 * n
   int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
+  int* jp;     // n: jpvt on the device during the DM loop
+  int* ncols;  // n: fjb_cmp of each DM step
   unsigned* tickets;  // ceil(n/64) counters of the trailing update (zero between uses)
   int nsplit;
 };
@@ -164,6 +169,8 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   t.Wp = carve<double>(p, (size_t)nsplit * KMAX * n);
   t.W = carve<double>(p, (size_t)KMAX * n);
   t.perm = carve<int>(p, kPermInts);
+  t.jp = carve<int>(p, n);
+  t.ncols = carve<int>(p, n);
   t.tickets = carve<unsigned>(p, (size_t)(n + 63) / 64 + 1);
   if (b) *b = t;
   return (size_t)(p - base);
@@ -281,7 +288,6 @@ int run_dm(const Args& x) {
   }
   if (x.nsteps) *x.nsteps = 0;
   const int minmn = m < n ? m : n;
-  const int kcap = nb < KMAX ? nb : KMAX;  // this device handles blocks up to 64
 
   Buffers B;
   Workspace& ws = workspace_for(m, n, &B);
@@ -331,146 +337,95 @@ int run_dm(const Args& x) {
       memcpy(&maxnrm, &bits, sizeof(double));
     }
     eta = eta * maxnrm;
-    std::vector<int> marked(n, 0);
-    std::vector<std::pair<int, int>> swaps;
-    std::vector<int> content(n), rank_of(n, -1);
-    for (int p = 0; p < n; p++) content[p] = p;
-    int j = nfxd, it = -1;
-    int n_fallback = 0;
+    // --- the DM loop (src/dgeqrdm_work.c:694-787), driven from the device:
+    // every kernel of a step reads j and the step's decisions from DevState,
+    // so the host queues kBatch steps at a time and synchronises once per
+    // batch; kernels of steps past the end return at once.  Host-side sizes
+    // use j0, the column reached at the last synchronisation (j >= j0).
     const char* pe = getenv("QRDM_PANEL");
     const bool force_exact = pe && strcmp(pe, "exact") == 0;
-    while (j < minmn) {
-      const int msub = m - j, nsub = n - j;
-      it++;
-      int kmax = nb;
-      if (msub < kmax) kmax = msub;
-      if (nsub < kmax) kmax = nsub;
-      if (kmax > kcap) kmax = kcap;
-      // --- DM_perm (src/dgeqrdm_work.c:280-418) -----------------------------
-      int ps = g_prof.begin(PC_SELECT, s);
-      launch_topk(B.work + j, nsub, kmax, tau_dm, B.topk_val, B.topk_idx, B.topk_half, B.st, s);
-      // candidate Gram over rows j..m (cosines need it only when nc > 1; the
-      // kernel reads nc from the state and zero-fills beyond it)
-      launch_gram(A + (size_t)j * lda, lda, j, m, B.st->cand_idx, kmax, &B.st->nc, B.gram_part,
-                  B.G, s);
-      launch_greedy(B.G, delta, kmax, B.st, s);
-      g_prof.end(ps, s, 8.0 * nsub + 8.0 * (double)msub * kmax);
+    const int fast_rows = force_exact ? INT_MAX : kFastPanelRows;
+    HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
+    launch_dm_begin(B.st, nfxd, minmn, s);
+    const int batch = x.trace ? 1 : kBatch;
+    int j0 = nfxd, it = 0;
+    while (j0 < minmn) {
+      for (int b = 0; b < batch; b++) {
+        // --- DM_perm (src/dgeqrdm_work.c:280-418) -------------------------
+        int ps = g_prof.begin(PC_SELECT, s);
+        launch_topk(B.work, j0, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.topk_half, B.st, s);
+        // candidate Gram over rows j..m (cosines need it only when nc > 1;
+        // the kernel reads nc from the state and zero-fills beyond it)
+        launch_gram(A, lda, j0, m, B.st->cand_idx, KMAX, &B.st->nc, B.gram_part, B.G, s, B.st,
+                    RUN_DM);
+        launch_select(B.G, delta, m, n, fast_rows, B.perm, B.st, s);
+        g_prof.end(ps, s, 0.0);
+        int pp = g_prof.begin(PC_PERM, s);
+        launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
+        g_prof.end(pp, s, 0.0);
+        // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) ----------------------
+        // Tall panels: Gram-based Householder reconstruction (panel.hip),
+        // which also leaves V and T for the trailing update; short or
+        // degenerate panels: the column-by-column exact panel, then dlarft.
+        int pq = g_prof.begin(PC_PANEL, s);
+        if (m - j0 >= kFastPanelRows && !force_exact)
+          launch_panel_fast(A, lda, j0, m, tau_dm, x.tau, B.G, B.perm + kRankOff, B.gram_part,
+                            B.R1, B.R1inv, B.G2, B.T, B.M, B.V, B.ldv, B.st, s);
+        launch_panel_exact(A, lda, j0, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
+        launch_build_v(A, lda, j0, m, B.ldv, B.st, B.V, B.ldv, s, RUN_EXACT);
+        launch_gram(B.V, B.ldv, j0, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s, B.st,
+                    RUN_EXACT);
+        launch_larft(B.S, x.tau, B.st, B.T, s, RUN_EXACT);
+        g_prof.end(pq, s, 0.0);
+        // --- dlarfb (src/dgeqrdm_work.c:762-767) from c0 (set by the panel)
+        int pl = g_prof.begin(PC_LARFB, s);
+        launch_larfb_w(A, lda, j0, m, j0 + 1, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit,
+                       B.tickets, s, B.st);
+        g_prof.end(pl, s, 0.0);
+        int pa = g_prof.begin(PC_APPLY, s);
+        launch_larfb_apply(A, lda, j0, m, j0 + 1, n, B.V, B.ldv, B.W, s, B.st);
+        g_prof.end(pa, s, 0.0);
+        // --- norm_update (src/dgeqrdm_work.c:774-778), stop test, next j ---
+        int pn = g_prof.begin(PC_NORM, s);
+        launch_norm_update(A, lda, j0, m, n, B.work, B.workold, B.st, tol3z, s);
+        launch_step_end(B.st, B.ncols, m, n, enable_stop, eta, s);
+        g_prof.end(pn, s, 0.0);
+      }
       HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
-      const int fjb = hst->fjb;
-      permute_walk(nsub, marked, hst->sel, fjb, swaps);
-      // compose the swap sequence into (dst, src) pairs over touched columns
-      std::vector<int> touched;
-      for (auto& pr : swaps) {
-        for (int q : {pr.first, pr.second}) {
-          if (content[q] == q && std::find(touched.begin(), touched.end(), q) == touched.end())
-            touched.push_back(q);
-        }
-        std::swap(content[pr.first], content[pr.second]);
-        std::swap(x.jpvt[j + pr.first], x.jpvt[j + pr.second]);
-      }
-      int npair = 0;
-      int* hp = reinterpret_cast<int*>(reinterpret_cast<char*>(ws.pinned) + sizeof(DevState));
-      for (int q : touched) {
-        if (content[q] != q) {
-          hp[2 * npair] = j + q;
-          hp[2 * npair + 1] = j + content[q];
-          npair++;
-        }
-      }
-      // candidate rank of each panel column (for the panel's Gram)
-      for (int q = 0; q < fjb; q++) rank_of[hst->sel[q]] = hst->selpos[q];
-      for (int q = 0; q < fjb && q < KMAX; q++) hp[kRankOff + q] = rank_of[content[q]];
-      for (int q = 0; q < fjb; q++) rank_of[hst->sel[q]] = -1;
-      for (int q : touched) content[q] = q;
-      HIPCHK(hipMemcpyAsync(B.perm, hp, sizeof(int) * kPermInts, hipMemcpyHostToDevice, s));
-      if (npair > 0) {
-        int pp = g_prof.begin(PC_PERM, s);
-        launch_apply_perm(A, lda, m, B.work, B.perm, npair, s);
-        g_prof.end(pp, s, 16.0 * (double)m * npair);
-      }
-      // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) --------------------------
-      // Tall panels: Gram-based Householder reconstruction (panel.hip), which
-      // also leaves V and T for the trailing update; short or degenerate
-      // panels: the column-by-column exact panel.
-      bool fast = fjb >= 2 && msub >= kFastPanelRows && !force_exact;
-      int pq = g_prof.begin(PC_PANEL, s);
-      if (fast) {
-        launch_panel_fast(A, lda, j, m, fjb, tau_dm, x.tau + j, B.G, B.perm + kRankOff,
-                          B.gram_part, B.R1, B.R1inv, B.G2, B.T, B.M, B.V, B.ldv, B.st, s);
-        HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, 4 * sizeof(int),
-                              hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (hst->flags & 2) {
-          fast = false;
-          launch_clear_flag(B.st, 2, s);
-          n_fallback++;
-        }
-      }
-      if (!fast) {
-        launch_panel_exact(A, lda, j, msub, fjb, tau_dm, 1, x.tau + j, B.st, s);
-        HIPCHK(hipMemcpyAsync(&hst->fjb_cmp, &B.st->fjb_cmp, sizeof(int), hipMemcpyDeviceToHost,
-                              s));
-        HIPCHK(hipStreamSynchronize(s));
-      }
-      g_prof.end(pq, s, 2.0 * msub * (double)fjb * fjb);
-      const int fjb_cmp = hst->fjb_cmp;
-      x.ncols[it] = fjb_cmp;
-      if (x.trace && it < x.ntrace) {
-        qrdm_step& r = x.trace[it];
+      if (x.trace && hst->it > it && hst->it - 1 < x.ntrace) {
+        const StepRec& l = hst->last;
+        qrdm_step& r = x.trace[hst->it - 1];
         memset(&r, 0, sizeof(r));
-        r.j = j;
-        r.kmax = kmax;
-        r.nc = hst->nc;
-        r.fjb = fjb;
-        r.fjb_cmp = fjb_cmp;
-        r.nswap = (int)swaps.size();
-        r.panel = fast ? 1 : 0;
-        for (int q = 0; q < fjb && q < 64; q++) r.sel[q] = hst->sel[q];
+        r.j = l.j;
+        r.kmax = l.kmax;
+        r.nc = l.nc;
+        r.fjb = l.fjb;
+        r.fjb_cmp = l.fjb_cmp;
+        r.nswap = l.nswap;
+        r.panel = l.panel;
+        r.maxnrm = l.maxnrm;
+        for (int q = 0; q < l.fjb && q < 64; q++) r.sel[q] = hst->sel[q];
       }
-      if (x.nsteps) *x.nsteps = it + 1;
-      // --- dlarft + dlarfb (src/dgeqrdm_work.c:751-767) ---------------------
-      if (!fast) {
-        int pt = g_prof.begin(PC_PANEL, s);
-        launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
-        launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
-        launch_larft(B.S, x.tau + j, B.st, B.T, s);
-        g_prof.end(pt, s, 1.0 * msub * (double)fjb_cmp * fjb_cmp);
-      }
-      // The exact panel already applied its reflectors to the rejected
-      // columns j+fjb_cmp..j+fjb-1 (dlarf_ inside dgeqr2_mia); the fast one
-      // leaves them to the trailing update.
-      const int c0 = fast ? j + fjb_cmp : j + fjb;
-      const double nct = (double)(n - c0);
-      int pl = g_prof.begin(PC_LARFB, s);
-      launch_larfb_w(A, lda, j, m, c0, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
-      g_prof.end(pl, s, 2.0 * msub * nct * fjb_cmp + 2.0 * fjb_cmp * (double)fjb_cmp * nct);
-      int pa = g_prof.begin(PC_APPLY, s);
-      launch_larfb_apply(A, lda, j, m, c0, n, B.V, B.ldv, B.W, s);
-      g_prof.end(pa, s, 16.0 * msub * nct + 8.0 * msub * fjb_cmp + 8.0 * fjb_cmp * nct);
-      // --- norm_update (src/dgeqrdm_work.c:774-778) -------------------------
-      HIPCHK(hipMemsetAsync(&B.st->maxnrm_bits, 0, sizeof(unsigned long long), s));
-      int pn = g_prof.begin(PC_NORM, s);
-      launch_norm_update(A, lda, j, m, n, B.work, B.workold, B.st, tol3z, s);
-      g_prof.end(pn, s, 8.0 * (double)(fjb_cmp + 2) * (nsub - fjb_cmp));
-      j += fjb_cmp;
-      if (enable_stop >= 1 || x.trace) {
-        HIPCHK(hipMemcpyAsync(&hst->maxnrm_bits, &B.st->maxnrm_bits, sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        unsigned long long bits = hst->maxnrm_bits;
-        memcpy(&maxnrm, &bits, sizeof(double));
-        if (x.trace && it < x.ntrace) x.trace[it].maxnrm = maxnrm;
-        if (enable_stop >= 1 && maxnrm * sqrt((double)(nsub - fjb_cmp)) <= eta) break;
-      }
+      it = hst->it;
+      if (hst->done) break;
+      j0 = hst->j;
     }
+    if (x.nsteps) *x.nsteps = it;
+    HIPCHK(hipMemcpyAsync(x.jpvt, B.jp, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    if (it > 0) HIPCHK(hipMemcpyAsync(x.ncols, B.ncols, sizeof(int) * it, hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
   if (g_prof.on) {
     g_prof.resolve();
-    HIPCHK(hipMemcpy(&hst->recomp_rows, &B.st->recomp_rows, sizeof(unsigned long long),
-                     hipMemcpyDeviceToHost));
-    g_prof.work[PC_NORM] += 8.0 * (double)hst->recomp_rows;
+    HIPCHK(hipMemcpy(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    const StepWork& w = hst->work;
+    g_prof.work[PC_SELECT] += w.select;
+    g_prof.work[PC_PERM] += w.perm;
+    g_prof.work[PC_PANEL] += w.panel;
+    g_prof.work[PC_LARFB] += w.gemm1;
+    g_prof.work[PC_APPLY] += w.gemm2;
+    g_prof.work[PC_NORM] += w.norm + 8.0 * (double)hst->recomp_rows;
   }
   return info;
 }
@@ -481,7 +436,7 @@ void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buf
                 hipStream_t s) {
   for (int j = 0; j < ncols_qr; j += KMAX) {
     const int kb = (ncols_qr - j) < KMAX ? (ncols_qr - j) : KMAX;
-    launch_panel_exact(A, lda, j, m - j, kb, 0.0, 0, tau + j, B.st, s);
+    launch_panel_exact(A, lda, j, m, kb, 0.0, 0, tau + j, B.st, s);
     launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
     launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
     launch_larft(B.S, tau + j, B.st, B.T, s);
@@ -575,6 +530,11 @@ const char* qrdm_version(void) { return "qrdm_amd 0.1 (gfx950)"; }
 void qrdm_profile_enable(int on) {
   std::lock_guard<std::mutex> lk(g_mu);
   g_prof.on = on != 0;
+}
+
+void qrdm_profile_select(unsigned mask) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_prof.mask = mask;
 }
 
 void qrdm_profile_reset(void) {

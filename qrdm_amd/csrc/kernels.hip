@@ -190,11 +190,18 @@ __device__ void bitonic_lds(double* v, int* ix) {
 
 // Stage 1: each workgroup sorts TOPK_CHUNK keys and keeps its first KMAX.
 // in_idx == nullptr means the key index is its position.
+// The first level (in_idx == nullptr) reads work[j..n) with j from st.
 __global__ __launch_bounds__(512) void k_topk_chunk(const double* __restrict__ in_val,
                                                     const int* __restrict__ in_idx, int n,
-                                                    double* out_val, int* out_idx) {
+                                                    double* out_val, int* out_idx,
+                                                    const DevState* st) {
   __shared__ double v[TOPK_CHUNK];
   __shared__ int ix[TOPK_CHUNK];
+  if (st->done) return;
+  if (!in_idx) {
+    in_val += st->j;
+    n -= st->j;
+  }
   const int base = blockIdx.x * TOPK_CHUNK;
   for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
     int g = base + t;
@@ -218,9 +225,20 @@ __global__ __launch_bounds__(512) void k_topk_chunk(const double* __restrict__ i
 // (> tau * max, src/dgeqrdm_work.c:348-351).
 __global__ __launch_bounds__(512) void k_topk_final(const double* __restrict__ in_val,
                                                     const int* __restrict__ in_idx, int n,
-                                                    int kmax, double tau, DevState* st) {
+                                                    int m_all, int n_all, int nb, double tau,
+                                                    DevState* st) {
   __shared__ double v[TOPK_CHUNK];
   __shared__ int ix[TOPK_CHUNK];
+  if (st->done) return;
+  const int j = st->j;
+  if (!in_idx) {
+    in_val += j;
+    n -= j;
+  }
+  // k_max = min(nb, m_sub, n_sub) (src/dgeqrdm_work.c:703), at most KMAX here
+  int kmax = nb < KMAX ? nb : KMAX;
+  if (m_all - j < kmax) kmax = m_all - j;
+  if (n_all - j < kmax) kmax = n_all - j;
   for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
     if (t < n) {
       v[t] = in_val[t];
@@ -243,27 +261,31 @@ __global__ __launch_bounds__(512) void k_topk_final(const double* __restrict__ i
     int nc = 0;
     while (nc < nk && v[nc] > lim) nc++;
     st->nc = nc;
+    st->kmax = kmax;
   }
 }
 
-void launch_topk(const double* keys, int n, int kmax, double tau, double* tmp_val,
+void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, double* tmp_val,
                  int* tmp_idx, int half, DevState* st, hipStream_t s) {
-  const double* cv = keys;
+  // chunk structure for n - j0 >= n - j keys; surplus slots hold -1 and sort last
+  const double* cv = work;
   const int* ci = nullptr;
-  int cn = n;
+  int cn = n - j0;
   double* ov = tmp_val;
   int* oi = tmp_idx;
+  int cnt = n;  // count passed to the kernels (first level: n, shifted by j inside)
   while (cn > TOPK_CHUNK) {
-    int nb = (cn + TOPK_CHUNK - 1) / TOPK_CHUNK;
-    hipLaunchKernelGGL(k_topk_chunk, dim3(nb), dim3(512), 0, s, cv, ci, cn, ov, oi);
+    int nbk = (cn + TOPK_CHUNK - 1) / TOPK_CHUNK;
+    hipLaunchKernelGGL(k_topk_chunk, dim3(nbk), dim3(512), 0, s, cv, ci, cnt, ov, oi, st);
     cv = ov;
     ci = oi;
-    cn = nb * KMAX;
+    cn = nbk * KMAX;
+    cnt = cn;
     // alternate between the two halves of the scratch lists
     ov = (ov == tmp_val) ? tmp_val + half : tmp_val;
     oi = (oi == tmp_idx) ? tmp_idx + half : tmp_idx;
   }
-  hipLaunchKernelGGL(k_topk_final, dim3(1), dim3(512), 0, s, cv, ci, cn, kmax, tau, st);
+  hipLaunchKernelGGL(k_topk_final, dim3(1), dim3(512), 0, s, cv, ci, cnt, m, n, nb, tau, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,9 +297,18 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
                                                       const int* __restrict__ cols,
                                                       int ncols_max,
                                                       const int* __restrict__ ncols_dev,
-                                                      double* partial) {
+                                                      double* partial, const DevState* st,
+                                                      int mode) {
   __shared__ double X[KMAX * (GRAM_ROWS + 1)];
+  if (!step_runs(st, mode)) return;
+  if (mode == RUN_DM) {  // the step's candidates: columns and rows from j
+    base += (size_t)st->j * ld;
+    r0 = st->j;
+  } else if (mode != RUN_ALWAYS) {
+    r0 = st->j;
+  }
   const int nc = ncols_dev ? *ncols_dev : ncols_max;
+  // slabs past r1 (the grid is sized for the lowest j) write zero partials
   const int rs = r0 + blockIdx.x * GRAM_ROWS;
   // 256 threads: 16 elements each, all loads issued before the LDS writes
   double xv[KMAX * GRAM_ROWS / 256];
@@ -317,8 +348,10 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
 }
 
 __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ partial,
-                                                     int nslab, double* G) {
+                                                     int nslab, double* G,
+                                                     const DevState* st, int mode) {
   __shared__ double part[4][64];
+  if (!step_runs(st, mode)) return;
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
   const int q = threadIdx.x >> 6;
   double s = 0.0;
@@ -330,69 +363,173 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ 
 }
 
 void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, int ncols_max,
-                 const int* ncols_dev, double* partial, double* G, hipStream_t s) {
+                 const int* ncols_dev, double* partial, double* G, hipStream_t s,
+                 const DevState* st, int mode) {
   int nslab = (r1 - r0 + GRAM_ROWS - 1) / GRAM_ROWS;
   if (nslab < 1) nslab = 1;
   hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, base, ld, r0, r1, cols,
-                     ncols_max, ncols_dev, partial);
-  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G);
+                     ncols_max, ncols_dev, partial, st, mode);
+  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
+                     st, mode);
 }
 
 // ---------------------------------------------------------------------------
-// Greedy acceptance (src/dgeqrdm_work.c:354-403): the max-norm column first,
-// then, in sorted order, every candidate whose |cos| with all earlier accepts
-// is < delta.  cos(p,q) = G(p,q) / (norm_p * norm_q) with the downdated norms
-// (the reference scales each candidate by 1/norm before its dsyrk).  One wave;
-// lane t tracks the running max |cos| of candidate t against the accepts.
-__global__ __launch_bounds__(64) void k_greedy(const double* __restrict__ G, double delta,
-                                               int kmax, DevState* st) {
+// The end of DM_perm (src/dgeqrdm_work.c:354-418), one wave:
+// 1. greedy acceptance: the max-norm column first, then, in sorted order,
+//    every candidate whose |cos| with all earlier accepts is < delta, while
+//    fewer than k_max are accepted.  cos(p,q) = G(p,q) / (norm_p * norm_q)
+//    with the downdated norms (the reference scales each candidate by 1/norm
+//    before its dsyrk).  Lane t tracks candidate t's running max |cos|; the
+//    loop is unrolled so lane t's Gram column stays in registers.
+// 2. permute_marked's walk (src/dgeqrdm_work.c:149-262, marks 0/1): the same
+//    two-pointer scan and placement order as the host walk permute_walk in
+//    driver.hip.  Lane s carries the position of mark s (marks travel with
+//    their columns), so "is position p marked" is one ballot; the composed
+//    effect of the swaps is kept as a map of at most 2 * KMAX touched
+//    positions (two entries per lane).
+// 3. the plan: (dst, src) pairs of the moved columns into perm[0..4*KMAX),
+//    the candidate rank of each panel column into perm[4*KMAX..5*KMAX), and
+//    the panel form (Gram-based when fjb >= 2 and m - j >= fast_rows).
+__global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, double delta,
+                                               int m, int n, int fast_rows, int* perm,
+                                               DevState* st) {
+  __shared__ int s_sel[KMAX], s_rank[KMAX];
+  if (st->done) return;
   const int t = threadIdx.x;
-  const int nc = st->nc;
-  if (t == 0) {
-    st->sel[0] = st->cand_idx[0];
-    st->selpos[0] = 0;
-  }
-  if (nc <= 1) {
-    if (t == 0) st->fjb = 1;
-    return;
-  }
-  const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
-  const double r0 = 1.0 / st->cand_val[0];
-  double run = (t < nc) ? fabs(G[0 + t * KMAX] * r0 * rt) : 0.0;
+  const int j = st->j, nc = st->nc, kmax = st->kmax;
+  const int nsub = n - j;
+  // --- 1. greedy
+  unsigned long long acc = 1ull;  // bit i: candidate i accepted
   int nsel = 1;
-  for (int i = 1; i < nc; i++) {
-    double mi = __shfl(run, i, 64);
-    if (mi < delta && nsel < kmax) {
-      if (t == 0) {
-        st->sel[nsel] = st->cand_idx[i];
-        st->selpos[nsel] = i;
+  if (nc > 1) {
+    double gc[KMAX];
+#pragma unroll
+    for (int i = 0; i < KMAX; i++) gc[i] = (t < nc) ? G[t + i * KMAX] : 0.0;  // G symmetric
+    const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
+    const double r0 = rdlane(rt, 0);
+    double run = (t < nc) ? fabs(gc[0] * r0 * rt) : 0.0;
+#pragma unroll
+    for (int i = 1; i < KMAX; i++) {
+      if (i < nc) {
+        const double mi = rdlane(run, i);
+        if (mi < delta && nsel < kmax) {
+          acc |= 1ull << i;
+          nsel++;
+          const double ri = rdlane(rt, i);
+          if (t > i && t < nc) run = fmax(run, fabs(gc[i] * ri * rt));
+        }
       }
-      nsel++;
-      double ri = __shfl(rt, i, 64);
-      if (t > i && t < nc) run = fmax(run, fabs(G[i + t * KMAX] * ri * rt));
     }
   }
-  if (t == 0) st->fjb = nsel;
+  const int fjb = nsel;
+  const bool mine = (acc >> t) & 1ull;
+  const int spos = __popcll(acc & ((1ull << t) - 1ull));  // acceptance order
+  const int cidx = st->cand_idx[t];
+  if (mine) {
+    s_sel[spos] = cidx;
+    s_rank[spos] = t;
+    st->sel[spos] = cidx;
+    st->selpos[spos] = t;
+  }
+  __syncthreads();
+  // --- 2. permute_marked's walk
+  int mp = (t < fjb) ? s_sel[t] : -1;  // position of mark t
+  int tp0 = -1, tc0 = 0, tp1 = -1, tc1 = 0;
+  int ntouch = 0, nswap = 0;
+  auto marked = [&](int p) -> bool { return __ballot(mp == p) != 0ull; };
+  auto entry = [&](int p) -> int {  // index of p in the touched map (added if absent)
+    const unsigned long long b0 = __ballot(tp0 == p);
+    if (b0) return __ffsll((long long)b0) - 1;
+    const unsigned long long b1 = __ballot(tp1 == p);
+    if (b1) return 64 + __ffsll((long long)b1) - 1;
+    const int e = ntouch++;
+    tp0 = (t == e) ? p : tp0;
+    tc0 = (t == e) ? p : tc0;
+    tp1 = (t + 64 == e) ? p : tp1;
+    tc1 = (t + 64 == e) ? p : tc1;
+    return e;
+  };
+  auto content = [&](int e) -> int {
+    const int x0 = __shfl(tc0, e & 63, 64), x1 = __shfl(tc1, e & 63, 64);
+    return e < 64 ? x0 : x1;
+  };
+  auto set_content = [&](int e, int v) {
+    tc0 = (t == e) ? v : tc0;
+    tc1 = (t + 64 == e) ? v : tc1;
+  };
+  auto swp = [&](int p, int q) {
+    mp = (mp == p) ? q : ((mp == q) ? p : mp);
+    const int ep = entry(p), eq = entry(q);
+    const int cp = content(ep), cq = content(eq);
+    set_content(ep, cq);
+    set_content(eq, cp);
+    nswap++;
+  };
+  int jb = 0, jt = nsub - 1;
+  for (int sidx = 0; sidx < fjb; sidx++) {
+    const int jc = s_sel[sidx];
+    while (jb < jt && marked(jt)) {
+      swp(jt, jb);
+      while (jb < nsub && marked(jb)) jb++;
+    }
+    if (marked(jc)) {
+      while (jb < nsub && marked(jb)) jb++;
+      if (jc <= jb || jc < fjb) continue;
+      if (!marked(jb)) {
+        swp(jc, jb);
+        jb++;
+      }
+    }
+  }
+  // --- 3. plan
+  const bool mv0 = t < ntouch && tp0 != tc0;
+  const bool mv1 = t + 64 < ntouch && tp1 != tc1;
+  const unsigned long long b0 = __ballot(mv0), b1 = __ballot(mv1);
+  const unsigned long long below = (1ull << t) - 1ull;
+  if (mv0) {
+    const int q = __popcll(b0 & below);
+    perm[2 * q] = j + tp0;
+    perm[2 * q + 1] = j + tc0;
+  }
+  if (mv1) {
+    const int q = __popcll(b0) + __popcll(b1 & below);
+    perm[2 * q] = j + tp1;
+    perm[2 * q + 1] = j + tc1;
+  }
+  // every accepted column now sits in [0, fjb); mark t rides with the
+  // column of acceptance rank t, candidate rank selpos[t]
+  if (t < fjb) perm[4 * KMAX + mp] = s_rank[t];
+  if (t == 0) {
+    st->fjb = fjb;
+    st->npair = __popcll(b0) + __popcll(b1);
+    st->nswap = nswap;
+    st->fast = (fjb >= 2 && m - j >= fast_rows) ? 1 : 0;
+  }
 }
 
-void launch_greedy(const double* G, double delta, int kmax, DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, s, G, delta, kmax, st);
+void launch_select(const double* G, double delta, int m, int n, int fast_rows, int* perm,
+                   DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, s, G, delta, m, n, fast_rows, perm, st);
 }
 
 // ---------------------------------------------------------------------------
 // Column permutation (the data movement of permute_marked,
-// src/dgeqrdm_work.c:149-262): perm holds npair (dst, src) pairs, a
+// src/dgeqrdm_work.c:149-262): perm holds st->npair (dst, src) pairs, a
 // permutation of the touched columns; column dst receives the old column src
-// (all m rows) and work[dst] the old work[src].  Each workgroup owns a row
-// slab, reads every source first and then writes, so no slab races another.
+// (all m rows), work[dst] the old work[src] and jp[dst] the old jp[src].
+// Each workgroup owns a row slab, reads every source first and then writes,
+// so no slab races another.
 constexpr int PERM_ROWS = 64;
 
 __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
-                                                    const int* __restrict__ perm, int npair) {
-  extern __shared__ double buf[];  // npair * PERM_ROWS
+                                                    int* jp, const int* __restrict__ perm,
+                                                    const DevState* st) {
+  __shared__ double buf[2 * KMAX * PERM_ROWS];
+  if (st->done) return;
+  const int npair = st->npair;
   const int rs = blockIdx.x * PERM_ROWS;
   // npair <= 2 * KMAX: at most 32 elements per thread (256 threads), loaded
-  // together before any store
+  // eight at a time before they are stored
   const int tot = npair * PERM_ROWS;
   for (int q0 = 0; q0 < 2 * KMAX * PERM_ROWS / 256; q0 += 8) {
     if (q0 * 256 >= tot) break;
@@ -410,20 +547,27 @@ __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, d
     }
   }
   double wv = 0.0;
-  if (blockIdx.x == 0 && threadIdx.x < npair) wv = work[perm[2 * threadIdx.x + 1]];
+  int jv = 0;
+  const bool side = blockIdx.x == 0 && threadIdx.x < npair;
+  if (side) {
+    wv = work[perm[2 * threadIdx.x + 1]];
+    if (jp) jv = jp[perm[2 * threadIdx.x + 1]];
+  }
   __syncthreads();
   for (int t = threadIdx.x; t < tot; t += 256) {
     const int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
     if (r < m) A[(size_t)perm[2 * p] * lda + r] = buf[t];
   }
-  if (blockIdx.x == 0 && threadIdx.x < npair) work[perm[2 * threadIdx.x]] = wv;
+  if (side) {
+    work[perm[2 * threadIdx.x]] = wv;
+    if (jp) jp[perm[2 * threadIdx.x]] = jv;
+  }
 }
 
-void launch_apply_perm(double* A, int lda, int m, double* work, const int* perm, int npair,
-                       hipStream_t s) {
-  if (npair <= 0) return;
-  hipLaunchKernelGGL(k_apply_perm, dim3((m + PERM_ROWS - 1) / PERM_ROWS), dim3(256),
-                     npair * PERM_ROWS * sizeof(double), s, A, lda, m, work, perm, npair);
+void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
+                       DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_perm, dim3((m + PERM_ROWS - 1) / PERM_ROWS), dim3(256), 0, s, A,
+                     lda, m, work, jp, perm, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -431,11 +575,19 @@ void launch_apply_perm(double* A, int lda, int m, double* work, const int* perm,
 // (src/dlarfg.c:116-185) and dlarf_ (src/dlarf.c:161-185), column by column
 // in one workgroup.  Truncation: for column i > 0 stop when ||x|| < thres,
 // thres = 5e-14, or tau_dm*|R(0,0)| once column 0 is done (tau_dm > 0).
-// truncate = 0 disables the test (unpivoted QR).
-__global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j, int msub,
+// truncate = 0 disables the test (unpivoted QR).  RUN_EXACT: j and fjb from
+// st, tau indexed from column 0 (the step writes tau[j..]).
+__global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j, int m,
                                                       int fjb, double tau_dm, int truncate,
-                                                      double* tau, DevState* st) {
+                                                      double* tau, DevState* st, int mode) {
   __shared__ double red[16];
+  if (!step_runs(st, mode)) return;
+  if (mode != RUN_ALWAYS) {
+    j = st->j;
+    fjb = st->fjb;
+    tau += j;
+  }
+  const int msub = m - j;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
   double* P = A + j + (size_t)j * lda;
   double thres = 5e-14;
@@ -490,13 +642,18 @@ __global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j,
       __syncthreads();
     }
   }
-  if (tid == 0) st->fjb_cmp = i;
+  if (tid == 0) {
+    st->fjb_cmp = i;
+    // dlarf_ inside the panel already updated the rejected columns
+    // j+fjb_cmp..j+fjb-1; the trailing update starts at j+fjb (:764-766)
+    st->c0 = j + fjb;
+  }
 }
 
-void launch_panel_exact(double* A, int lda, int j, int msub, int fjb, double tau_dm,
-                        int truncate, double* tau, DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_panel_exact, dim3(1), dim3(1024), 0, s, A, lda, j, msub, fjb, tau_dm,
-                     truncate, tau, st);
+void launch_panel_exact(double* A, int lda, int j, int m, int fjb, double tau_dm,
+                        int truncate, double* tau, DevState* st, hipStream_t s, int mode) {
+  hipLaunchKernelGGL(k_panel_exact, dim3(1), dim3(1024), 0, s, A, lda, j, m, fjb, tau_dm,
+                     truncate, tau, st, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,7 +661,9 @@ void launch_panel_exact(double* A, int lda, int j, int msub, int fjb, double tau
 // KMAX columns and to mpad rows, zero above row j) into V[r + c*ldv].
 __global__ __launch_bounds__(256) void k_build_v(const double* __restrict__ A, int lda, int j,
                                                  int m, int mpad, const DevState* st,
-                                                 double* V, int ldv) {
+                                                 double* V, int ldv, int mode) {
+  if (!step_runs(st, mode)) return;
+  if (mode != RUN_ALWAYS) j = st->j;
   const int kk = st->fjb_cmp;
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= mpad) return;
@@ -517,9 +676,9 @@ __global__ __launch_bounds__(256) void k_build_v(const double* __restrict__ A, i
 }
 
 void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevState* st,
-                    double* V, int ldv, hipStream_t s) {
+                    double* V, int ldv, hipStream_t s, int mode) {
   hipLaunchKernelGGL(k_build_v, dim3((mpad + 255) / 256), dim3(256), 0, s, A, lda, j, m, mpad,
-                     st, V, ldv);
+                     st, V, ldv, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,9 +686,11 @@ void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevS
 // T(0:i,i) = -tau_i T(0:i,0:i) (V^T v_i)(0:i), T(i,i) = tau_i, from S = V^T V.
 __global__ __launch_bounds__(64) void k_larft(const double* __restrict__ S,
                                               const double* __restrict__ tau,
-                                              const DevState* st, double* T) {
+                                              const DevState* st, double* T, int mode) {
   __shared__ double Ts[KMAX * KMAX];
   __shared__ double tmp[KMAX];
+  if (!step_runs(st, mode)) return;
+  if (mode != RUN_ALWAYS) tau += st->j;
   const int p = threadIdx.x;
   const int kk = st->fjb_cmp;
   for (int c = 0; c < KMAX; c++) Ts[p + c * KMAX] = 0.0;
@@ -552,8 +713,8 @@ __global__ __launch_bounds__(64) void k_larft(const double* __restrict__ S,
 }
 
 void launch_larft(const double* S, const double* tau, const DevState* st, double* T,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, S, tau, st, T);
+                  hipStream_t s, int mode) {
+  hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, S, tau, st, T, mode);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,16 +762,27 @@ __device__ __forceinline__ void load_rows4(const double* __restrict__ col, int r
   }
 }
 
+// With st (the DM loop) j, c0 come from st and ncol = ldw - c0; column tiles
+// past ncol return before touching their ticket (the grid is sized for the
+// lowest j), row slabs past m contribute zero partials.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
                                                int m, int c0, int ncol,
                                                const double* __restrict__ V, int ldv,
                                                int slab, double* Wp, int ldw,
                                                const double* __restrict__ T, int trans,
-                                               double* W, unsigned* tickets) {
+                                               double* W, unsigned* tickets,
+                                               const DevState* st) {
   __shared__ double Ts[KMAX][KMAX + 1];
   __shared__ double Ws[KMAX][KMAX + 1];
   __shared__ int s_last;
+  if (st) {
+    if (st->done) return;
+    j = st->j;
+    c0 = st->c0;
+    ncol = ldw - c0;
+  }
+  if ((int)blockIdx.x * 64 >= ncol) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int base = j & ~(G1_CHUNK - 1);
@@ -782,8 +954,16 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
 // shared through LDS.
 __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m, int c0,
                                                int ncol, const double* __restrict__ V, int ldv,
-                                               const double* __restrict__ W, int ldw) {
+                                               const double* __restrict__ W, int ldw,
+                                               const DevState* st) {
   __shared__ double Vs[64][KMAX + 1];
+  if (st) {  // the DM loop: geometry of the current step, grid sized for the lowest j
+    if (st->done) return;
+    j = st->j;
+    c0 = st->c0;
+    ncol = ldw - c0;
+  }
+  if (j + (int)blockIdx.x * 64 >= m || (int)blockIdx.y * 64 >= ncol) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int rbase = j + blockIdx.x * 64;
@@ -845,7 +1025,8 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
 
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                     const double* T, int trans, double* Wp, double* W, int nsplit,
-                    unsigned* tickets, hipStream_t s) {
+                    unsigned* tickets, hipStream_t s, const DevState* st) {
+  // with st: j, c0 are lower bounds (c0 >= j + 1 then) of the step's values
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ldw = n;
@@ -867,26 +1048,26 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
                       ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
   if (aligned)
     hipLaunchKernelGGL(k_gemm1<true>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol, V,
-                       ldv, slab, Wp, ldw, T, trans, W, tickets);
+                       ldv, slab, Wp, ldw, T, trans, W, tickets, st);
   else
     hipLaunchKernelGGL(k_gemm1<false>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
-                       V, ldv, slab, Wp, ldw, T, trans, W, tickets);
+                       V, ldv, slab, Wp, ldw, T, trans, W, tickets, st);
 }
 
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                        const double* W, hipStream_t s) {
+                        const double* W, hipStream_t s, const DevState* st) {
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ctiles = (ncol + 63) / 64;
   hipLaunchKernelGGL(k_gemm2, dim3((mrow + 63) / 64, ctiles), dim3(256), 0, s, A, lda, j, m, c0,
-                     ncol, V, ldv, W, n);
+                     ncol, V, ldv, W, n, st);
 }
 
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                   const double* T, int trans, double* Wp, double* W, int nsplit,
                   unsigned* tickets, hipStream_t s) {
-  launch_larfb_w(A, lda, j, m, c0, n, V, ldv, T, trans, Wp, W, nsplit, tickets, s);
-  launch_larfb_apply(A, lda, j, m, c0, n, V, ldv, W, s);
+  launch_larfb_w(A, lda, j, m, c0, n, V, ldv, T, trans, Wp, W, nsplit, tickets, s, nullptr);
+  launch_larfb_apply(A, lda, j, m, c0, n, V, ldv, W, s, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -896,9 +1077,11 @@ void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double*
 // reset.  workold is indexed by position and is not permuted, as in the
 // reference.  One wave per column.
 __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ A, int lda,
-                                                     int j, int m, int n, double* work,
+                                                     int m, int n, double* work,
                                                      double* workold, DevState* st,
                                                      double tol3z) {
+  if (st->done) return;
+  const int j = st->j;
   const int kk = st->fjb_cmp;
   const int col = j + kk + blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -929,12 +1112,72 @@ __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ 
   }
 }
 
-void launch_norm_update(const double* A, int lda, int j, int m, int n, double* work,
+void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* work,
                         double* workold, DevState* st, double tol3z, hipStream_t s) {
-  const int ncols = n - j;  // upper bound; the kernel skips the first fjb_cmp
+  const int ncols = n - j0;  // upper bound; the kernel starts at j + fjb_cmp
   if (ncols <= 0) return;
-  hipLaunchKernelGGL(k_norm_update, dim3((ncols + 3) / 4), dim3(256), 0, s, A, lda, j, m, n,
-                     work, workold, st, tol3z);
+  hipLaunchKernelGGL(k_norm_update, dim3((ncols + 3) / 4), dim3(256), 0, s, A, lda, m, n, work,
+                     workold, st, tol3z);
+}
+
+// ---------------------------------------------------------------------------
+// End of a DM step (src/dgeqrdm_work.c:740, :774-785): ncols[it] = fjb_cmp,
+// the step's record and algorithmic work, j += fjb_cmp, the stop test
+// maxnrm * sqrt(n_sub - fjb_cmp) <= eta, and the loop's end at j >= min(m,n).
+__global__ void k_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop,
+                           double eta) {
+  if (st->done) return;
+  const int j = st->j, kk = st->fjb_cmp, fjb = st->fjb;
+  double maxnrm;
+  {
+    const unsigned long long b = st->maxnrm_bits;
+    maxnrm = __longlong_as_double((long long)b);
+  }
+  const int msub = m - j, nsub = n - j;
+  StepRec& r = st->last;
+  r.j = j;
+  r.kmax = st->kmax;
+  r.nc = st->nc;
+  r.fjb = fjb;
+  r.fjb_cmp = kk;
+  r.nswap = st->nswap;
+  r.panel = (st->fast && !(st->flags & 2)) ? 1 : 0;
+  r.maxnrm = maxnrm;
+  const double nct = (double)(n - st->c0);
+  StepWork& w = st->work;
+  w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
+  w.perm += 16.0 * (double)m * st->npair;
+  w.panel += 2.0 * msub * (double)fjb * fjb;
+  w.gemm1 += 2.0 * msub * nct * kk + 2.0 * kk * (double)kk * nct;
+  w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
+  w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
+  ncols_dev[st->it] = kk;
+  st->it += 1;
+  st->j = j + kk;
+  const int minmn = m < n ? m : n;
+  int done = (j + kk >= minmn) ? 1 : 0;
+  if (enable_stop && maxnrm * sqrt((double)(nsub - kk)) <= eta) done = 1;
+  st->done = done;
+  st->maxnrm_bits = 0ull;
+  st->flags &= ~2;
+}
+
+void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s, st, ncols_dev, m, n, enable_stop, eta);
+}
+
+__global__ void k_dm_begin(DevState* st, int j, int minmn) {
+  st->j = j;
+  st->it = 0;
+  st->done = (j >= minmn) ? 1 : 0;
+  st->maxnrm_bits = 0ull;
+  st->flags &= ~2;
+  st->work = StepWork{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+}
+
+void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s) {
+  hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(1), 0, s, st, j, minmn);
 }
 
 // ---------------------------------------------------------------------------
@@ -951,8 +1194,10 @@ void launch_clear_flag(DevState* st, int bits, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, st, bits);
 }
 
-void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s) {
-  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G);
+void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s,
+                        const DevState* st, int mode) {
+  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
+                     st, mode);
 }
 
 // Swap two whole columns (cblas_dswap of the fixed-column pre-pass,

@@ -47,14 +47,6 @@ constexpr int BLK = 16;        // block width of the one-workgroup factorisation
 
 typedef double dbl4p __attribute__((ext_vector_type(4)));
 
-// Value of x in lane l (l a compile-time or wave-uniform index).
-__device__ __forceinline__ double rdlane(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)b, l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 // S[r][c] -= sum_{t < 16} A(r, t) B(t, c) for r, c in [lo, nlim), 16 x 16
 // tiles on the matrix cores (wave w takes tiles w, w + 4, ...).
 // sym: A(r, t) = S[bs + t][r] and only tiles on or above the diagonal (the
@@ -349,15 +341,16 @@ __device__ __forceinline__ void tri_inv_upper(const double (*Rin)[LDP], bool tra
 // the rank map), stopped at the first pivot that proves the DM truncation
 // fires there; R1^-1.  Identity padding outside fjb.
 __global__ __launch_bounds__(256) void k_chol1(const double* __restrict__ Gc,
-                                               const int* __restrict__ rank, int fjb,
-                                               double tau_dm, double* R1, double* R1inv,
-                                               DevState* st) {
+                                               const int* __restrict__ rank, double tau_dm,
+                                               double* R1, double* R1inv, DevState* st) {
   __shared__ double S[KMAX][LDP];
   __shared__ double X[KMAX][LDP];
   __shared__ double Y[3][BLK][BLK + 1];
   __shared__ double rinv[KMAX], diag[KMAX];
   __shared__ int srk[KMAX];
   __shared__ int s_done, s_why;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int fjb = st->fjb;
   const int tid = threadIdx.x;
   if (tid < KMAX) {
     const int rk = tid < fjb ? rank[tid] : 0;
@@ -478,12 +471,14 @@ __device__ __forceinline__ void load_mat(double (*S)[LDP], const double* __restr
 }
 
 // Pass 2: Q = P R1^-1 on a 64-row slab, partial Gram Q^T Q.
-__global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int lda, int j,
-                                               int m, const double* __restrict__ R1inv,
+// Slabs past m (the grid is sized for the lowest j) write zero partials.
+__global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int lda, int m,
+                                               const double* __restrict__ R1inv,
                                                const DevState* st, double* partial) {
   __shared__ double Ps[PR_ROWS][LDP];  // P slab (row, col), then Q
   __shared__ double Ri[KMAX][LDP];
-  if (st->flags & 2) return;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int j = st->j;
   const int k1 = st->k1;
   const int tid = threadIdx.x;
   const int rs = j + blockIdx.x * PR_ROWS;
@@ -511,9 +506,8 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
 // fallback flag.  st->fjb_cmp = k.
 __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
                                                const double* __restrict__ R1, double* A,
-                                               int lda, int j, int fjb, double tau_dm,
-                                               double* tau, double* T_out, double* M_out,
-                                               DevState* st) {
+                                               int lda, double tau_dm, double* tau,
+                                               double* T_out, double* M_out, DevState* st) {
   __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
   __shared__ double SA[KMAX][LDP];  // R^-1
   __shared__ double SW[KMAX][LDP];  // P_top, then Q11 and its LU, then L^-T
@@ -521,7 +515,9 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
   __shared__ double Y[3][BLK][BLK + 1];
   __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
   __shared__ int s_done, s_why;
-  if (st->flags & 2) return;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int j = st->j, fjb = st->fjb;
+  tau += j;
   const int tid = threadIdx.x;
   const int k1 = st->k1;
   {
@@ -627,22 +623,30 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
         M_out[r + c * KMAX] = (r < k && c < k) ? acc[u][v] : 0.0;
       }
   }
-  if (tid == 0) st->fjb_cmp = k;
+  if (tid == 0) {
+    st->fjb_cmp = k;
+    // the trailing update also applies the kept reflectors to the rejected
+    // in-block columns j+k..j+fjb-1 (dlarf_ did that inside the reference's
+    // panel)
+    st->c0 = j + k;
+  }
 }
 
 // ---------------------------------------------------------------------------
 // Pass 3: V2 = P_bottom M on a 64-row slab; writes V2 into the panel below the
 // top block and the dense zero-padded V (rows [j, m), KMAX columns) with V1's
 // unit lower block on top.
-__global__ __launch_bounds__(256) void k_v2(double* A, int lda, int j, int m,
+__global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
                                             const double* __restrict__ M, const DevState* st,
                                             double* V, int ldv) {
   __shared__ double Ps[PR_ROWS][LDP];
   __shared__ double Ms[KMAX][LDP];
-  if (st->flags & 2) return;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int j = st->j;
   const int k = st->fjb_cmp;
   const int tid = threadIdx.x;
   const int rs = j + blockIdx.x * PR_ROWS;
+  if (rs >= m) return;
   load_slab(Ps, A, lda, j, rs, m, k);
   load_mat(Ms, M);
   __syncthreads();
@@ -678,17 +682,16 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int j, int m,
   }
 }
 
-void launch_panel_fast(double* A, int lda, int j, int m, int fjb, double tau_dm, double* tau,
+void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
                        const double* Gc, const int* rank, double* gram_part, double* R1,
                        double* R1inv, double* G2, double* T, double* M, double* V, int ldv,
                        DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_chol1, dim3(1), dim3(256), 0, s, Gc, rank, fjb, tau_dm, R1, R1inv, st);
-  const int nslab = (m - j + PR_ROWS - 1) / PR_ROWS;
-  hipLaunchKernelGGL(k_qgram, dim3(nslab), dim3(256), 0, s, A, lda, j, m, R1inv, st, gram_part);
-  launch_gram_reduce(gram_part, nslab, G2, s);
-  hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, G2, R1, A, lda, j, fjb, tau_dm, tau, T,
-                     M, st);
-  hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, j, m, M, st, V, ldv);
+  hipLaunchKernelGGL(k_chol1, dim3(1), dim3(256), 0, s, Gc, rank, tau_dm, R1, R1inv, st);
+  const int nslab = (m - j0 + PR_ROWS - 1) / PR_ROWS;
+  hipLaunchKernelGGL(k_qgram, dim3(nslab), dim3(256), 0, s, A, lda, m, R1inv, st, gram_part);
+  launch_gram_reduce(gram_part, nslab, G2, s, st, RUN_FAST);
+  hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, G2, R1, A, lda, tau_dm, tau, T, M, st);
+  hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, m, M, st, V, ldv);
 }
 
 }  // namespace qrdm

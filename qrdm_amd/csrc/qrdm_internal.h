@@ -10,8 +10,22 @@ constexpr int KMAX = 64;          // largest DM block the kernels handle
 constexpr int TOPK_CHUNK = 1024;  // keys sorted per workgroup by the top-k stage
 constexpr int GRAM_ROWS = 64;     // rows per workgroup of the Gram kernel
 
-// Small per-factorisation state shared by the kernels of one step.  Lives in
-// device memory; the host reads it back between kernels where it must.
+// Record of one completed DM step (kept for tracing).
+struct StepRec {
+  int j, kmax, nc, fjb, fjb_cmp, nswap, panel;
+  double maxnrm;
+};
+
+// Algorithmic work of the steps run so far, per kernel class (flops or HBM
+// bytes), accumulated on the device for the kernel-time accounting.
+struct StepWork {
+  double select, perm, panel, gemm1, gemm2, norm;
+};
+
+// Per-factorisation state shared by the kernels.  Lives in device memory.
+// The DM loop is driven from the device: every kernel of a step reads the
+// step's column j (and what earlier kernels of the step decided) from here,
+// so the host queues steps without waiting for them.
 struct DevState {
   int nc;          // number of norm candidates
   int fjb;         // accepted columns
@@ -24,50 +38,111 @@ struct DevState {
   int selpos[KMAX];     // their candidate ranks
   int cand_idx[KMAX];   // candidates (relative to j), sorted
   double cand_val[KMAX];
+  int j;       // first column of the current step
+  int it;      // steps completed
+  int done;    // 1: the loop has ended; kernels of later steps return at once
+  int kmax;    // min(nb, m - j, n - j, KMAX) of the current step
+  int fast;    // 1: the current step's panel is the Gram-based one
+  int c0;      // first column of the current step's trailing update
+  int npair;   // (dst, src) column pairs of the current step's permutation
+  int nswap;   // swaps made by the current step's permutation walk
+  StepRec last;
+  StepWork work;
 };
 
+// Value of x in lane l of the wave (l a compile-time or wave-uniform index).
+__device__ __forceinline__ double rdlane(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Which kernels of a step run: host-driven callers (RUN_ALWAYS) pass their
+// own geometry; the DM loop's kernels (RUN_DM) take j from DevState and
+// return once the loop is done; RUN_EXACT / RUN_FAST kernels belong to one
+// of the two panel forms and return unless the step uses that form.
+enum RunMode { RUN_ALWAYS = 0, RUN_DM = 1, RUN_EXACT = 2, RUN_FAST = 3 };
+
+__device__ __forceinline__ bool step_runs(const DevState* st, int mode) {
+  if (mode == RUN_ALWAYS) return true;
+  if (st->done) return false;
+  if (mode == RUN_EXACT) return !st->fast || (st->flags & 2);
+  if (mode == RUN_FAST) return st->fast && !(st->flags & 2);
+  return true;
+}
+
 // ---- launchers (kernels.hip) ------------------------------------------------
+// Kernels that take a RunMode read j (and the other per-step values named
+// below) from st when mode != RUN_ALWAYS; the host arguments are then upper
+// bounds used only to size the grid.
 void launch_init_norms(const double* A, int lda, int r0, int m, int n, double* work,
                        double* workold, DevState* st, hipStream_t s);
-// Sorted top-k of keys[0..n) (value descending, index ascending).  tmp_val/
-// tmp_idx hold 2 * half entries, half >= ceil(n/TOPK_CHUNK)*KMAX.
-void launch_topk(const double* keys, int n, int kmax, double tau, double* tmp_val,
+// Sorted top-kmax of work[j..n) (value descending, index ascending) and the
+// candidate count (> tau * max), into st.  j, kmax = min(nb, m-j, n-j, KMAX)
+// from st.  tmp_val/tmp_idx hold 2 * half entries, half >=
+// ceil(n/TOPK_CHUNK)*KMAX.  j0 <= j sizes the grid.
+void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, double* tmp_val,
                  int* tmp_idx, int half, DevState* st, hipStream_t s);
 // Gram of ncols (<= 64) columns over rows [r0, r1): columns are
 // base[(cols ? cols[c] : c) * ld + r].  partial needs ceil((r1-r0)/GRAM_ROWS)
 // * 64 * 64 doubles.  G is 64 x 64 column-major; entries beyond ncols are 0.
+// mode RUN_DM: base and r0 move by st->j columns and rows (the candidates of
+// the step); RUN_EXACT: r0 = st->j (the step's V), run only with the exact
+// panel.  r0 is then the lower bound j0.
 void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, int ncols_max,
-                 const int* ncols_dev, double* partial, double* G, hipStream_t s);
-void launch_greedy(const double* G, double delta, int kmax, DevState* st, hipStream_t s);
-void launch_apply_perm(double* A, int lda, int m, double* work, const int* perm, int npair,
-                       hipStream_t s);
-void launch_panel_exact(double* A, int lda, int j, int msub, int fjb, double tau_dm,
-                        int truncate, double* tau, DevState* st, hipStream_t s);
+                 const int* ncols_dev, double* partial, double* G, hipStream_t s,
+                 const DevState* st = nullptr, int mode = RUN_ALWAYS);
+// Greedy cosine acceptance, then the step's plan: permute_marked's walk, the
+// (dst, src) column pairs into perm[0..4*KMAX), the panel columns' candidate
+// ranks into perm[4*KMAX..5*KMAX), the panel form.
+void launch_select(const double* G, double delta, int m, int n, int fast_rows, int* perm,
+                   DevState* st, hipStream_t s);
+// Moves the step's columns (st->npair pairs in perm), their work entries and
+// jpvt entries (jp may be null).
+void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
+                       DevState* st, hipStream_t s);
+// Column-by-column panel on columns [j, j + fjb) (RUN_ALWAYS: host j, fjb;
+// RUN_EXACT: from st, tau indexed from j).  Sets st->fjb_cmp and st->c0.
+void launch_panel_exact(double* A, int lda, int j, int m, int fjb, double tau_dm,
+                        int truncate, double* tau, DevState* st, hipStream_t s,
+                        int mode = RUN_ALWAYS);
 void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevState* st,
-                    double* V, int ldv, hipStream_t s);
+                    double* V, int ldv, hipStream_t s, int mode = RUN_ALWAYS);
 void launch_larft(const double* S, const double* tau, const DevState* st, double* T,
-                  hipStream_t s);
+                  hipStream_t s, int mode = RUN_ALWAYS);
 // Block reflector from the left on rows [j, m), columns [c0, n):
 // trans = 1: C := (I - V T V^T)^T C (dgeqrdm's trailing update);
 // trans = 0: C := (I - V T V^T) C.
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                   const double* T, int trans, double* Wp, double* W, int nsplit,
                   unsigned* tickets, hipStream_t s);
-// the two halves of launch_larfb: W = op(T) V^T C, then C -= V W
+// The two halves of launch_larfb: W = op(T) V^T C, then C -= V W.  With st
+// (the DM loop) j and c0 come from st and the host values are lower bounds.
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                     const double* T, int trans, double* Wp, double* W, int nsplit,
-                    unsigned* tickets, hipStream_t s);
+                    unsigned* tickets, hipStream_t s, const DevState* st = nullptr);
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                        const double* W, hipStream_t s);
-void launch_norm_update(const double* A, int lda, int j, int m, int n, double* work,
+                        const double* W, hipStream_t s, const DevState* st = nullptr);
+// Norm downdate of columns [j + fjb_cmp, n) (j, fjb_cmp from st; j0 <= j
+// sizes the grid).
+void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* work,
                         double* workold, DevState* st, double tol3z, hipStream_t s);
+// End of a DM step: ncols, the step record and work, the stop test
+// (enable_stop, eta absolute), the next j, done.
+void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
+                     hipStream_t s);
+// Start of the DM loop at column j.
+void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
-void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s);
-// Tall-panel DM QR without per-column grid reductions (panel.hip).  Gc is
-// the step's candidate Gram (64 x 64, candidate order) and rank[p] the
-// candidate rank of panel column p.
-void launch_panel_fast(double* A, int lda, int j, int m, int fjb, double tau_dm, double* tau,
+void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s,
+                        const DevState* st = nullptr, int mode = RUN_ALWAYS);
+// Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST
+// kernels: j, fjb from st.  Gc is the step's candidate Gram (64 x 64,
+// candidate order) and rank[p] the candidate rank of panel column p.  j0 <=
+// j sizes the grids.  Sets st->fjb_cmp and st->c0, or the fallback flag.
+void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
                        const double* Gc, const int* rank, double* gram_part, double* R1,
                        double* R1inv, double* G2, double* T, double* M, double* V, int ldv,
                        DevState* st, hipStream_t s);
