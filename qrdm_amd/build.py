@@ -17,9 +17,14 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed FP32 instructions are kept out of every device code object.
+# Packed FP32 instructions are kept out of every device code object.  The
+# matrix-core accumulators are allocated in ordinary VGPRs (gfx950 reads and
+# writes them from either register file); with the AGPR form the compiler
+# copied every accumulator between the two files on each pass of the
+# trailing update's main loop.
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-Wall",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-mfma-vgpr-form=1"]
 LIB = os.path.join(HERE, "libqrdm_amd.so")
 
 

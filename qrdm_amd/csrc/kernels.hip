@@ -721,46 +721,15 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // Trailing update (LAPACKE_dlarfb_mia, src/dlarfb.c:142 -> dlarfb 'L','T','F',
 // 'C'): C := C - V (C^T V T)^T on rows [j, m), columns [c0, n); with
 // trans = 0 the block reflector itself, C := C - V T V^T C.
-//   gemm1:   Wp[s] = V(rows of slab s)^T C(rows of slab s)  (split over rows)
-//   wreduce: W = T^T sum_s Wp[s]
-//   gemm2:   C -= V W
+//   k_gemm1: W = op(T) V^T C, the rows split in slabs; each workgroup writes
+//            its slab's partial V^T C, the last one of a column tile (atomic
+//            ticket) sums the partials in slab order and applies op(T);
+//   k_gemm2: C -= V W.
 // Matrix cores: v_mfma_f64_16x16x4f64, D(16x16) += A(16x4) B(4x16); lane l
 // holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4*v][l%16], v = 0..3 (the
 // output rows of one lane are interleaved by 4, as measured on gfx950).
-
-// k_gemm1: W = op(T) (V^T C) with the reduction over rows split in slabs.
-// Rows are walked in aligned 32-row chunks; within a chunk the MFMA k index
-// of lane group g = lane/16 at step s is row 8g + s, so every lane reads 8
-// consecutive rows of one column (two 32-byte loads) for both operands.  The
-// V chunk (32 rows x 64 columns) is shared by the 4 waves through LDS.  Each
-// workgroup writes its slab's partial; the last one of a column tile (atomic
-// ticket) sums the partials in slab order and applies T^T (or T).
 constexpr int G1_CHUNK = 32;
 constexpr int G1_MAX_SPLIT = 8;  // row slabs of the split reduction (Wp holds this many)
-
-__device__ __forceinline__ void load_rows8(const double* __restrict__ col, int r, int rlo, int rhi,
-                                           double (&out)[8]) {
-  if (r >= rlo && r + 8 <= rhi && ((reinterpret_cast<uintptr_t>(col + r) & 31) == 0)) {
-    const dbl4 x = *reinterpret_cast<const dbl4*>(col + r);
-    const dbl4 y = *reinterpret_cast<const dbl4*>(col + r + 4);
-    out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; out[3] = x[3];
-    out[4] = y[0]; out[5] = y[1]; out[6] = y[2]; out[7] = y[3];
-  } else {
-#pragma unroll
-    for (int q = 0; q < 8; q++) out[q] = (r + q >= rlo && r + q < rhi) ? col[r + q] : 0.0;
-  }
-}
-
-__device__ __forceinline__ void load_rows4(const double* __restrict__ col, int r, int rlo, int rhi,
-                                           double (&out)[4]) {
-  if (r >= rlo && r + 4 <= rhi && ((reinterpret_cast<uintptr_t>(col + r) & 31) == 0)) {
-    const dbl4 x = *reinterpret_cast<const dbl4*>(col + r);
-    out[0] = x[0]; out[1] = x[1]; out[2] = x[2]; out[3] = x[3];
-  } else {
-#pragma unroll
-    for (int q = 0; q < 4; q++) out[q] = (r + q >= rlo && r + q < rhi) ? col[r + q] : 0.0;
-  }
-}
 
 // With st (the DM loop) j, c0 come from st and ncol = ldw - c0; column tiles
 // past ncol return before touching their ticket (the grid is sized for the
@@ -787,8 +756,8 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
   const int g = lane >> 4, li = lane & 15;
   const int base = j & ~(G1_CHUNK - 1);
   const int rs = base + blockIdx.y * slab;
-  const int re = min(rs + slab, m);
-  const int rlo = max(rs, j);
+  // a slab past m (the grid is sized for the lowest j) is empty: re = rs
+  const int re = max(min(rs + slab, m), rs);
   const int cb = blockIdx.x * 64;
   // every wave accumulates the whole 64 x 64 tile over chunks wave, wave+4,
   // ...  Columns past ncol read a valid column instead: their output columns
@@ -807,15 +776,17 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
   for (int t = 0; t < 4; t++)
 #pragma unroll
     for (int u = 0; u < 4; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  // 16-row chunks, wave w taking chunks w, w+4, ...; lane group g holds rows
-  // 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  Every chunk issues
-  // the same eight 32-byte loads (row index clamped into the column); rows
-  // outside [rlo, re) are zeroed when used.  With a fixed load count per
-  // chunk the next chunk's loads stay in flight during this chunk's MFMAs.
-  constexpr int CH = 16, STRIDE = 4 * CH;
+  // 16-row chunks of the slab, wave w taking chunks w, w+4, ...; lane group
+  // g holds rows 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  V is
+  // zero on the rows [base, j) above the panel (the V builders keep it so),
+  // so only the slab's last chunk, which may run past m, needs masking.  The
+  // full chunks go through a two-buffer pipeline whose loads are issued a
+  // chunk ahead of the MFMAs that consume the previous one.
+  constexpr int CH = 16;
+  const int nfull = (re - rs) >> 4;  // chunks entirely below re (re <= m)
+  const int nmine = nfull > wave ? (nfull - wave + 3) >> 2 : 0;
   // last 4-row group that starts inside the column: with lda % 4 == 0 (the
-  // vector path's requirement) rows up to round_up(m, 4) - 1 exist in memory,
-  // and a group holding any row < m never needs clamping
+  // vector path's requirement) rows up to round_up(m, 4) - 1 exist in memory
   const int rclamp = ((m + 3) & ~3) - 4;
   auto load_chunk = [&](int r, dbl4 (&cc)[4], dbl4 (&vc)[4]) {
     if constexpr (VEC) {
@@ -835,48 +806,71 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
       }
     }
   };
-  auto mfma_chunk = [&](int r, const dbl4 (&cc)[4], const dbl4 (&vc)[4]) {
-    const int rr = r + 4 * g;
+  auto mfma_chunk = [&](const dbl4 (&cc)[4], const dbl4 (&vc)[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vc[t][q], cc[u][q], acc[t][u], 0, 0, 0);
+  };
+  auto chunk_row = [&](int i) { return rs + CH * (wave + 4 * i); };
+  dbl4 ca[4], va[4];
+  if (nmine > 0) load_chunk(chunk_row(0), ca, va);
+#pragma clang loop unroll(disable)
+  for (int i = 0; i < nmine; i++) {
+    // chunk i + 1 (clamped to the last one, then unused) is in flight while
+    // chunk i goes through the matrix cores
+    dbl4 cn[4], vn[4];
+    load_chunk(chunk_row(min(i + 1, nmine - 1)), cn, vn);
+    __builtin_amdgcn_sched_barrier(0);
+    mfma_chunk(ca, va);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      ca[u] = cn[u];
+      va[u] = vn[u];
+    }
+  }
+  // the partial chunk at the end of the column, rows >= re masked
+  if (rs + CH * nfull < re && (nfull & 3) == wave) {
+    const int r = rs + CH * nfull;
+    load_chunk(r, ca, va);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const bool ok = rr + q >= rlo && rr + q < re;
+      const bool ok = r + 4 * g + q < re;
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        const double a = ok ? vc[t][q] : 0.0;
+        const double a = ok ? va[t][q] : 0.0;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const double b = ok ? cc[u][q] : 0.0;
+          const double b = ok ? ca[u][q] : 0.0;
           acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t][u], 0, 0, 0);
         }
       }
     }
-  };
-  // chunks are processed in pairs with a fixed trip count; an odd count runs
-  // one fully masked chunk (all its rows are >= re and contribute zero)
-  dbl4 ca[4], va[4], cn[4], vn[4];
-  int r0 = rs + CH * wave;
-  const int npair = r0 < re ? (re - r0 + 2 * STRIDE - 1) / (2 * STRIDE) : 0;
-  if (npair > 0) load_chunk(r0, ca, va);
-  for (int it = 0; it < npair; it++) {
-    load_chunk(r0 + STRIDE, cn, vn);
-    mfma_chunk(r0, ca, va);
-    load_chunk(r0 + 2 * STRIDE, ca, va);
-    mfma_chunk(r0 + STRIDE, cn, vn);
-    r0 += 2 * STRIDE;
   }
   // sum the four waves' tiles in wave order: row (V column) 16t + g + 4v,
   // column 16u + li
-  for (int w = 0; w < 4; w++) {
+  if (wave == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Ws[16 * t + g + 4 * v][16 * u + li] = acc[t][u][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int w = 1; w < 4; w++) {
     if (wave == w) {
 #pragma unroll
       for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int u = 0; u < 4; u++)
 #pragma unroll
-          for (int v = 0; v < 4; v++) {
-            double& d = Ws[16 * t + g + 4 * v][16 * u + li];
-            d = (w == 0) ? acc[t][u][v] : d + acc[t][u][v];
-          }
+          for (int v = 0; v < 4; v++) Ws[16 * t + g + 4 * v][16 * u + li] += acc[t][u][v];
     }
     __syncthreads();
   }

@@ -647,6 +647,15 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
   const int tid = threadIdx.x;
   const int rs = j + blockIdx.x * PR_ROWS;
   if (rs >= m) return;
+  if (blockIdx.x == 0) {
+    // V is zero above the panel: the trailing update's slabs start at the
+    // 32-row boundary at or below j and rely on it
+    const int z0 = j >= KMAX ? j - KMAX : 0;
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = z0 + (e & 63), c = e >> 6;
+      if (r < j) V[(size_t)c * ldv + r] = 0.0;
+    }
+  }
   load_slab(Ps, A, lda, j, rs, m, k);
   load_mat(Ms, M);
   __syncthreads();
