@@ -147,7 +147,7 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   const int mpad = ((m + 63) / 64) * 64;
   const int nslab = (m + 31) / 32 + 1;  // slabs of the Gram and fast-panel passes
   const int half = ((n + TOPK_CHUNK - 1) / TOPK_CHUNK) * KMAX + KMAX;
-  const int nsplit = 8;
+  const int nsplit = 16;  // row slabs of the trailing update's first product
   Buffers t;
   t.st = carve<DevState>(p, 1);
   t.work = carve<double>(p, n);

@@ -729,166 +729,165 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // holds A[l%16][l/16], B[l/16][l%16] and D[l/16 + 4*v][l%16], v = 0..3 (the
 // output rows of one lane are interleaved by 4, as measured on gfx950).
 constexpr int G1_CHUNK = 32;
-constexpr int G1_MAX_SPLIT = 8;  // row slabs of the split reduction (Wp holds this many)
+constexpr int G1_MAX_SPLIT = 16;  // row slabs of the split reduction (Wp holds this many)
 
-// With st (the DM loop) j, c0 come from st and ncol = ldw - c0; column tiles
-// past ncol return before touching their ticket (the grid is sized for the
-// lowest j), row slabs past m contribute zero partials.
+// k_gemm1: the partial products Wp[y] = V^T C over row slab y.  A workgroup
+// covers 256 columns of C: wave w owns the 64-column tile 4*blockIdx.x + w and
+// the four waves walk the same 16-row chunks of the slab, so the V chunk
+// (16 rows x 64 columns) is loaded once per workgroup into LDS (double
+// buffered, one barrier per chunk) while each wave streams its own C tile
+// through registers one chunk ahead of its MFMAs.  Lane group g = lane/16
+// holds rows 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  Only the
+// slab's last chunk can run past m; its rows >= re are masked.  Each wave
+// writes its 64 x 64 partial tile straight from its accumulators.
+// With st (the DM loop) j, c0 come from st and ncol = ldw - c0; the grid is
+// sized for the lowest j, so whole workgroups past ncol return and row slabs
+// past m write zero partials.
+constexpr int G1_VPAD = 4;  // LDS row padding of the V chunk (keeps 32-byte alignment)
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
                                                int m, int c0, int ncol,
                                                const double* __restrict__ V, int ldv,
                                                int slab, double* Wp, int ldw,
-                                               const double* __restrict__ T, int trans,
-                                               double* W, unsigned* tickets,
                                                const DevState* st) {
-  __shared__ double Ts[KMAX][KMAX + 1];
-  __shared__ double Ws[KMAX][KMAX + 1];
-  __shared__ int s_last;
+  __shared__ __align__(32) double Vs[2][KMAX][16 + G1_VPAD];
   if (st) {
     if (st->done) return;
     j = st->j;
     c0 = st->c0;
     ncol = ldw - c0;
   }
-  if ((int)blockIdx.x * 64 >= ncol) return;
+  if ((int)blockIdx.x * 256 >= ncol) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int base = j & ~(G1_CHUNK - 1);
   const int rs = base + blockIdx.y * slab;
-  // a slab past m (the grid is sized for the lowest j) is empty: re = rs
-  const int re = max(min(rs + slab, m), rs);
-  const int cb = blockIdx.x * 64;
-  // every wave accumulates the whole 64 x 64 tile over chunks wave, wave+4,
-  // ...  Columns past ncol read a valid column instead: their output columns
-  // are never stored.
+  const int re = max(min(rs + slab, m), rs);  // a slab past m is empty
+  const int cbw = blockIdx.x * 256 + wave * 64;  // this wave's tile (relative to c0)
+  // columns past ncol read a valid column instead; they are never stored
   const double* Cc[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const int c = cb + 16 * u + li;
+    const int c = cbw + 16 * u + li;
     Cc[u] = A + (size_t)(c0 + (c < ncol ? c : ncol - 1)) * lda;
   }
-  const double* Vt[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++) Vt[t] = V + (size_t)(16 * t + li) * ldv;
   dbl4 acc[4][4];
 #pragma unroll
   for (int t = 0; t < 4; t++)
 #pragma unroll
     for (int u = 0; u < 4; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  // 16-row chunks of the slab, wave w taking chunks w, w+4, ...; lane group
-  // g holds rows 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  V is
-  // zero on the rows [base, j) above the panel (the V builders keep it so),
-  // so only the slab's last chunk, which may run past m, needs masking.  The
-  // full chunks go through a two-buffer pipeline whose loads are issued a
-  // chunk ahead of the MFMAs that consume the previous one.
   constexpr int CH = 16;
-  const int nfull = (re - rs) >> 4;  // chunks entirely below re (re <= m)
-  const int nmine = nfull > wave ? (nfull - wave + 3) >> 2 : 0;
+  const int nch = (re - rs + CH - 1) / CH;  // the last chunk may end past re
   // last 4-row group that starts inside the column: with lda % 4 == 0 (the
   // vector path's requirement) rows up to round_up(m, 4) - 1 exist in memory
   const int rclamp = ((m + 3) & ~3) - 4;
-  auto load_chunk = [&](int r, dbl4 (&cc)[4], dbl4 (&vc)[4]) {
+  // V chunk loader: thread tid brings rows 4*(tid&3).. of V column tid>>2
+  const int vcol = tid >> 2, vq = 4 * (tid & 3);
+  const double* Vcol = V + (size_t)vcol * ldv;
+  auto load_v = [&](int c) -> dbl4 {
+    const int r = rs + CH * c + vq;
+    dbl4 v;
     if constexpr (VEC) {
-      const int rr = min(r + 4 * g, rclamp);
+      v = *reinterpret_cast<const dbl4*>(Vcol + min(r, rclamp));
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] = Vcol[min(r + q, m - 1)];
+    }
+    return v;
+  };
+  // rows >= re (past m, where V may hold anything) are stored as zero
+  auto store_v = [&](int buf, int c, dbl4 v) {
+    const int r = rs + CH * c + vq;
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = (r + q < re) ? v[q] : 0.0;
+    *reinterpret_cast<dbl4*>(&Vs[buf][vcol][vq]) = v;
+  };
+  auto load_c = [&](int c, dbl4 (&cc)[4]) {
+    const int r = rs + CH * c + 4 * g;
+    if constexpr (VEC) {
+      const int rr = min(r, rclamp);
 #pragma unroll
       for (int u = 0; u < 4; u++) cc[u] = *reinterpret_cast<const dbl4*>(Cc[u] + rr);
-#pragma unroll
-      for (int t = 0; t < 4; t++) vc[t] = *reinterpret_cast<const dbl4*>(Vt[t] + rr);
     } else {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        const int rq = min(r + 4 * g + q, m - 1);
+        const int rq = min(r + q, m - 1);
 #pragma unroll
         for (int u = 0; u < 4; u++) cc[u][q] = Cc[u][rq];
-#pragma unroll
-        for (int t = 0; t < 4; t++) vc[t][q] = Vt[t][rq];
       }
     }
   };
-  auto mfma_chunk = [&](const dbl4 (&cc)[4], const dbl4 (&vc)[4]) {
+  dbl4 ca[4];
+  if (nch > 0) {
+    store_v(0, 0, load_v(0));
+    load_c(0, ca);
+  }
+  __syncthreads();
+#pragma clang loop unroll(disable)
+  for (int c = 0; c < nch; c++) {
+    const int buf = c & 1;
+    // chunk c + 1 (clamped to the last chunk, then unused) in flight
+    const int cn_idx = min(c + 1, nch - 1);
+    const dbl4 vnext = load_v(cn_idx);
+    dbl4 cn[4];
+    load_c(cn_idx, cn);
+    dbl4 vc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) vc[t] = *reinterpret_cast<const dbl4*>(&Vs[buf][16 * t + li][4 * g]);
+    if (rs + CH * c + CH > re) {
+      // the chunk runs past re: its C rows there may be anything (padding)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const bool ok = rs + CH * c + 4 * g + q < re;
+#pragma unroll
+        for (int u = 0; u < 4; u++) ca[u][q] = ok ? ca[u][q] : 0.0;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int q = 0; q < 4; q++)
 #pragma unroll
       for (int t = 0; t < 4; t++)
 #pragma unroll
         for (int u = 0; u < 4; u++)
-          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vc[t][q], cc[u][q], acc[t][u], 0, 0, 0);
-  };
-  auto chunk_row = [&](int i) { return rs + CH * (wave + 4 * i); };
-  dbl4 ca[4], va[4];
-  if (nmine > 0) load_chunk(chunk_row(0), ca, va);
-#pragma clang loop unroll(disable)
-  for (int i = 0; i < nmine; i++) {
-    // chunk i + 1 (clamped to the last one, then unused) is in flight while
-    // chunk i goes through the matrix cores
-    dbl4 cn[4], vn[4];
-    load_chunk(chunk_row(min(i + 1, nmine - 1)), cn, vn);
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vc[t][q], ca[u][q], acc[t][u], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    mfma_chunk(ca, va);
-    __builtin_amdgcn_sched_barrier(0);
+    store_v(buf ^ 1, cn_idx, vnext);
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      ca[u] = cn[u];
-      va[u] = vn[u];
-    }
+    for (int u = 0; u < 4; u++) ca[u] = cn[u];
+    __syncthreads();
   }
-  // the partial chunk at the end of the column, rows >= re masked
-  if (rs + CH * nfull < re && (nfull & 3) == wave) {
-    const int r = rs + CH * nfull;
-    load_chunk(r, ca, va);
+  // partial tile: W row (V column) 16t + g + 4v, column cbw + 16u + li
+  double* out = Wp + (size_t)blockIdx.y * KMAX * ldw;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const bool ok = r + 4 * g + q < re;
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const double a = ok ? va[t][q] : 0.0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const double b = ok ? ca[u][q] : 0.0;
-          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t][u], 0, 0, 0);
-        }
-      }
-    }
-  }
-  // sum the four waves' tiles in wave order: row (V column) 16t + g + 4v,
-  // column 16u + li
-  if (wave == 0) {
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) Ws[16 * t + g + 4 * v][16 * u + li] = acc[t][u][v];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int w = 1; w < 4; w++) {
-    if (wave == w) {
+  for (int u = 0; u < 4; u++) {
+    const int cc = cbw + 16 * u + li;
+    if (cc < ncol) {
 #pragma unroll
       for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) Ws[16 * t + g + 4 * v][16 * u + li] += acc[t][u][v];
-    }
-    __syncthreads();
-  }
-  {
-    double* out = Wp + (size_t)blockIdx.y * KMAX * ldw;
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-      const int i = e >> 6, cc = cb + (e & 63);
-      if (cc < ncol) out[(size_t)i * ldw + cc] = Ws[i][e & 63];
+        for (int v = 0; v < 4; v++) out[(size_t)(16 * t + g + 4 * v) * ldw + cc] = acc[t][u][v];
     }
   }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) s_last = (atomicAdd(&tickets[blockIdx.x], 1u) == gridDim.y - 1);
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // last workgroup of this column tile: sum the partials in slab order into
-  // LDS (each element read once, coalesced), then apply op(T)
+}
+
+// k_wsum: W = op(T) sum_y Wp[y] for one 64-column tile per workgroup; the
+// partials are summed in slab order (deterministic), op(T) applied on the
+// matrix cores (T upper triangular, zero below).  trans = 1: T^T.
+__global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wp, int ns, int ldw,
+                                              int c0, int ncol, const double* __restrict__ T,
+                                              int trans, double* W, const DevState* st) {
+  __shared__ double Ts[KMAX][KMAX + 1];
+  __shared__ double Ws[KMAX][KMAX + 1];
+  if (st) {
+    if (st->done) return;
+    c0 = st->c0;
+    ncol = ldw - c0;
+  }
+  const int cb = blockIdx.x * 64;
+  if (cb >= ncol) return;
+  const int tid = threadIdx.x;
   {
     double tv[KMAX * KMAX / 256];
 #pragma unroll
@@ -899,9 +898,6 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
       Ts[e & 63][e >> 6] = tv[q];
     }
   }
-  // slab partials summed in slab order; every load of an element is issued
-  // before the sum (slab index clamped, the surplus terms masked)
-  const int ny = gridDim.y;
 #pragma unroll 4
   for (int q = 0; q < KMAX * KMAX / 256; q++) {
     const int e = tid + 256 * q;
@@ -910,35 +906,34 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
     double pv[G1_MAX_SPLIT];
 #pragma unroll
     for (int y = 0; y < G1_MAX_SPLIT; y++)
-      pv[y] = Wp[((size_t)(y < ny ? y : ny - 1) * KMAX + i) * ldw + ccl];
+      pv[y] = Wp[((size_t)(y < ns ? y : ns - 1) * KMAX + i) * ldw + ccl];
     double sum = 0.0;
 #pragma unroll
-    for (int y = 0; y < G1_MAX_SPLIT; y++) sum += (y < ny) ? pv[y] : 0.0;
-    Ws[i][e & 63] = (cc < ncol) ? sum : 0.0;
+    for (int y = 0; y < G1_MAX_SPLIT; y++) sum += (y < ns) ? pv[y] : 0.0;
+    Ws[i][e & 63] = sum;
   }
   __syncthreads();
-  const int k0 = 4 * (tid >> 4), q0 = 4 * (tid & 15);
-  double out[4][4] = {};
-  for (int i = 0; i < KMAX; i++) {
-    double wi[4], tv[4];
+  // out(r, c) = sum_i op(T)(r, i) Ws(i, c); wave w rows 16w..16w+15
+  const int lane = tid & 63, wv = tid >> 6, li = lane & 15, kq = lane >> 4;
+  dbl4 acc[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) wi[b] = Ws[i][q0 + b];
+  for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  const int ri = 16 * wv + li;
+#pragma unroll 4
+  for (int k0 = 0; k0 < KMAX; k0 += 4) {
+    const double av = trans ? Ts[k0 + kq][ri] : Ts[ri][k0 + kq];
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const int k = k0 + a;
-      tv[a] = trans ? ((i <= k) ? Ts[i][k] : 0.0) : ((i >= k) ? Ts[k][i] : 0.0);
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) out[a][b] = fma(tv[a], wi[b], out[a][b]);
+    for (int u = 0; u < 4; u++)
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Ws[k0 + kq][16 * u + li], acc[u], 0, 0, 0);
   }
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int u = 0; u < 4; u++) {
+    const int cc = cb + 16 * u + li;
+    if (cc < ncol) {
 #pragma unroll
-    for (int b = 0; b < 4; b++)
-      if (cb + q0 + b < ncol) W[(size_t)(k0 + a) * ldw + cb + q0 + b] = out[a][b];
-  if (tid == 0) tickets[blockIdx.x] = 0u;
+      for (int v = 0; v < 4; v++) W[(size_t)(16 * wv + kq + 4 * v) * ldw + cc] = acc[u][v];
+    }
+  }
 }
 
 // k_gemm2: C -= V W computed as C^T -= W^T V^T on the matrix cores, so that
@@ -1020,14 +1015,14 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                     const double* T, int trans, double* Wp, double* W, int nsplit,
                     unsigned* tickets, hipStream_t s, const DevState* st) {
+  (void)tickets;
   // with st: j, c0 are lower bounds (c0 >= j + 1 then) of the step's values
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ldw = n;
-  const int ctiles = (ncol + 63) / 64;
-  // enough workgroups to fill the chip: split the rows when there are few
-  // column tiles
-  int want = (1024 + ctiles - 1) / ctiles;
+  const int cgroups = (ncol + 255) / 256;
+  // about one workgroup per CU: split the rows when there are few column groups
+  int want = (256 + cgroups - 1) / cgroups;
   if (want > nsplit) want = nsplit;
   if (want > G1_MAX_SPLIT) want = G1_MAX_SPLIT;
   const int base = j & ~(G1_CHUNK - 1);
@@ -1041,11 +1036,13 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
                       ((reinterpret_cast<uintptr_t>(A) & 31) == 0) &&
                       ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
   if (aligned)
-    hipLaunchKernelGGL(k_gemm1<true>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol, V,
-                       ldv, slab, Wp, ldw, T, trans, W, tickets, st);
+    hipLaunchKernelGGL(k_gemm1<true>, dim3(cgroups, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
+                       V, ldv, slab, Wp, ldw, st);
   else
-    hipLaunchKernelGGL(k_gemm1<false>, dim3(ctiles, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
-                       V, ldv, slab, Wp, ldw, T, trans, W, tickets, st);
+    hipLaunchKernelGGL(k_gemm1<false>, dim3(cgroups, ns), dim3(256), 0, s, A, lda, j, m, c0,
+                       ncol, V, ldv, slab, Wp, ldw, st);
+  hipLaunchKernelGGL(k_wsum, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, ns, ldw, c0, ncol, T,
+                     trans, W, st);
 }
 
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
