@@ -137,24 +137,53 @@ __device__ __forceinline__ double dsign(double a, double b) {
 // ---------------------------------------------------------------------------
 // Initial partial norms: work[c] = workold[c] = ||A(r0:m, c)||, max over all
 // columns (src/dgeqrdm_work.c:670-682).  One wave per column.
+// One wave per column, NORM_COLS columns per wave; the maximum is reduced in
+// the workgroup first so that only one atomic per workgroup reaches the
+// single maxnrm word (same-address atomics serialise).
+constexpr int NORM_COLS = 4;
+
+__device__ __forceinline__ void block_max_atomic(double v, int bad, DevState* st) {
+  __shared__ double wmax[4];
+  __shared__ int wbad[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    wmax[wave] = v;
+    wbad[wave] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    const int b = wbad[0] | wbad[1] | wbad[2] | wbad[3];
+    if (b) atomicOr(&st->flags, 1);
+    if (mx > 0.0) atomic_max_nonneg(&st->maxnrm_bits, mx);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_init_norms(const double* __restrict__ A, int lda,
                                                      int r0, int m, int n, double* work,
                                                      double* workold, DevState* st) {
-  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (col >= n) return;
-  double nr = wave_nrm2(A + (size_t)col * lda + r0, m - r0);
-  if ((threadIdx.x & 63) == 0) {
-    work[col] = nr;
-    workold[col] = nr;
-    if (!(nr <= DBL_MAX)) atomicOr(&st->flags, 1);
-    else atomic_max_nonneg(&st->maxnrm_bits, nr);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double mx = 0.0;
+  int bad = 0;
+  for (int q = 0; q < NORM_COLS; q++) {
+    const int col = (blockIdx.x * 4 + wave) * NORM_COLS + q;
+    if (col >= n) break;
+    const double nr = wave_nrm2(A + (size_t)col * lda + r0, m - r0);
+    if (lane == 0) {
+      work[col] = nr;
+      workold[col] = nr;
+    }
+    if (!(nr <= DBL_MAX)) bad = 1;
+    else mx = fmax(mx, nr);
   }
+  block_max_atomic(mx, bad, st);
 }
 
 void launch_init_norms(const double* A, int lda, int r0, int m, int n, double* work,
                        double* workold, DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_init_norms, dim3((n + 3) / 4), dim3(256), 0, s, A, lda, r0, m, n, work,
-                     workold, st);
+  const int waves = (n + NORM_COLS - 1) / NORM_COLS;
+  hipLaunchKernelGGL(k_init_norms, dim3((waves + 3) / 4), dim3(256), 0, s, A, lda, r0, m, n,
+                     work, workold, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,7 +548,7 @@ void launch_select(const double* G, double delta, int m, int n, int fast_rows, i
 // (all m rows), work[dst] the old work[src] and jp[dst] the old jp[src].
 // Each workgroup owns a row slab, reads every source first and then writes,
 // so no slab races another.
-constexpr int PERM_ROWS = 64;
+constexpr int PERM_ROWS = 16;  // rows per workgroup: m / 16 workgroups move the columns
 
 __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
                                                     int* jp, const int* __restrict__ perm,
@@ -528,8 +557,8 @@ __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, d
   if (st->done) return;
   const int npair = st->npair;
   const int rs = blockIdx.x * PERM_ROWS;
-  // npair <= 2 * KMAX: at most 32 elements per thread (256 threads), loaded
-  // eight at a time before they are stored
+  // npair <= 2 * KMAX: at most 8 elements per thread (256 threads), all
+  // loaded before they are stored
   const int tot = npair * PERM_ROWS;
   for (int q0 = 0; q0 < 2 * KMAX * PERM_ROWS / 256; q0 += 8) {
     if (q0 * 256 >= tot) break;
@@ -1071,35 +1100,43 @@ __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ 
                                                      int m, int n, double* work,
                                                      double* workold, DevState* st,
                                                      double tol3z) {
+  __shared__ unsigned long long s_rows[4];
   if (st->done) return;
   const int j = st->j;
   const int kk = st->fjb_cmp;
-  const int col = j + kk + blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (col >= n) return;
-  double wk = work[col];
-  if (wk == 0.0) return;
-  const double* a = A + (size_t)col * lda + j;
-  const double x = (lane < kk) ? a[lane] : 0.0;
-  const double s = wave_sum(x * x);
-  double temp = sqrt(fabs(s)) / wk;
-  const double d2 = (temp + 1.0) * (1.0 - temp);
-  temp = (0.0 >= d2) ? 0.0 : d2;
-  const double d1 = wk / workold[col];
-  const double temp2 = temp * (d1 * d1);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int mrem = m - j - kk;
-  if (temp2 <= tol3z) {
-    wk = (mrem > 0) ? wave_nrm2(a + kk, mrem) : 0.0;
-    if (lane == 0) {
-      workold[col] = wk;
-      atomicAdd(&st->recomp_rows, (unsigned long long)(mrem > 0 ? mrem : 0));
+  double mx = 0.0;
+  unsigned long long rows = 0;
+  for (int q = 0; q < NORM_COLS; q++) {
+    const int col = j + kk + (blockIdx.x * 4 + wave) * NORM_COLS + q;
+    if (col >= n) break;
+    double wk = work[col];
+    if (wk == 0.0) continue;
+    const double* a = A + (size_t)col * lda + j;
+    // naive sum of squares of the kk new R rows (src/dgeqrdm_work.c:84-86)
+    const double x = (lane < kk) ? a[lane] : 0.0;
+    const double s = wave_sum(x * x);
+    double temp = sqrt(fabs(s)) / wk;
+    const double d2 = (temp + 1.0) * (1.0 - temp);
+    temp = (0.0 >= d2) ? 0.0 : d2;
+    const double d1 = wk / workold[col];
+    const double temp2 = temp * (d1 * d1);
+    if (temp2 <= tol3z) {
+      wk = (mrem > 0) ? wave_nrm2(a + kk, mrem) : 0.0;
+      if (lane == 0) workold[col] = wk;
+      rows += (unsigned long long)(mrem > 0 ? mrem : 0);
+    } else {
+      wk *= sqrt(temp);
     }
-  } else {
-    wk *= sqrt(temp);
+    if (lane == 0) work[col] = wk;
+    if (wk == wk) mx = fmax(mx, wk);
   }
-  if (lane == 0) {
-    work[col] = wk;
-    if (wk == wk) atomic_max_nonneg(&st->maxnrm_bits, wk);
+  if (lane == 0) s_rows[wave] = rows;
+  block_max_atomic(mx, 0, st);
+  if (threadIdx.x == 0) {
+    const unsigned long long r = s_rows[0] + s_rows[1] + s_rows[2] + s_rows[3];
+    if (r) atomicAdd(&st->recomp_rows, r);
   }
 }
 
@@ -1107,7 +1144,8 @@ void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* 
                         double* workold, DevState* st, double tol3z, hipStream_t s) {
   const int ncols = n - j0;  // upper bound; the kernel starts at j + fjb_cmp
   if (ncols <= 0) return;
-  hipLaunchKernelGGL(k_norm_update, dim3((ncols + 3) / 4), dim3(256), 0, s, A, lda, m, n, work,
+  const int waves = (ncols + NORM_COLS - 1) / NORM_COLS;
+  hipLaunchKernelGGL(k_norm_update, dim3((waves + 3) / 4), dim3(256), 0, s, A, lda, m, n, work,
                      workold, st, tol3z);
 }
 
