@@ -195,26 +195,44 @@ __device__ __forceinline__ bool key_before(double va, int ia, double vb, int ib)
   return va > vb || (va == vb && ia < ib);
 }
 
-// Bitonic sort of N (power of two) keys in LDS, descending in key_before order.
+// Orders a wave's LDS accesses: its earlier stores are visible to its later
+// loads (one wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bitonic sort of N (power of two) keys in LDS, descending in key_before
+// order, by N/2 threads (thread t compares one pair per stage).  A stage whose
+// compare distance jj is at most 64 keeps every pair of wave w inside the
+// keys [128w, 128w + 128), so it needs only the wave's own ordering; the
+// stages with jj >= 128 are separated by workgroup barriers.
 template <int N>
 __device__ void bitonic_lds(double* v, int* ix) {
+  const int t = threadIdx.x;
   for (int k = 2; k <= N; k <<= 1) {
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      for (int t = threadIdx.x; t < N / 2; t += blockDim.x) {
-        int lo = 2 * t - (t & (jj - 1));
-        int hi = lo + jj;
-        bool up = ((lo & k) == 0);
-        double va = v[lo], vb = v[hi];
-        int ia = ix[lo], ib = ix[hi];
-        bool sw = up ? key_before(vb, ib, va, ia) : key_before(va, ia, vb, ib);
-        if (sw) {
-          v[lo] = vb; v[hi] = va;
-          ix[lo] = ib; ix[hi] = ia;
-        }
+      const int lo = 2 * t - (t & (jj - 1));
+      const int hi = lo + jj;
+      const bool up = ((lo & k) == 0);
+      const double va = v[lo], vb = v[hi];
+      const int ia = ix[lo], ib = ix[hi];
+      const bool sw = up ? key_before(vb, ib, va, ia) : key_before(va, ia, vb, ib);
+      if (sw) {
+        v[lo] = vb;
+        v[hi] = va;
+        ix[lo] = ib;
+        ix[hi] = ia;
       }
-      __syncthreads();
+      if (jj > 64 || (jj == 1 && k < N && (k << 1) > 128)) {
+        __syncthreads();
+      } else {
+        wave_lds_sync();
+      }
     }
   }
+  __syncthreads();
 }
 
 // Stage 1: each workgroup sorts TOPK_CHUNK keys and keeps its first KMAX.
@@ -408,45 +426,43 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 //    every candidate whose |cos| with all earlier accepts is < delta, while
 //    fewer than k_max are accepted.  cos(p,q) = G(p,q) / (norm_p * norm_q)
 //    with the downdated norms (the reference scales each candidate by 1/norm
-//    before its dsyrk).  Lane t tracks candidate t's running max |cos|; the
-//    loop is unrolled so lane t's Gram column stays in registers.
-// 2. permute_marked's walk (src/dgeqrdm_work.c:149-262, marks 0/1): the same
-//    two-pointer scan and placement order as the host walk permute_walk in
-//    driver.hip.  Lane s carries the position of mark s (marks travel with
-//    their columns), so "is position p marked" is one ballot; the composed
-//    effect of the swaps is kept as a map of at most 2 * KMAX touched
-//    positions (two entries per lane).
+//    before its dsyrk).
+// 2. permute_marked's walk (src/dgeqrdm_work.c:149-262, marks 0/1): its
+//    placement (the host walk permute_walk in driver.hip) in closed form.
 // 3. the plan: (dst, src) pairs of the moved columns into perm[0..4*KMAX),
 //    the candidate rank of each panel column into perm[4*KMAX..5*KMAX), and
 //    the panel form (Gram-based when fjb >= 2 and m - j >= fast_rows).
 __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, double delta,
                                                int m, int n, int fast_rows, int* perm,
                                                DevState* st) {
+  __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
   if (st->done) return;
   const int t = threadIdx.x;
   const int j = st->j, nc = st->nc, kmax = st->kmax;
   const int nsub = n - j;
-  // --- 1. greedy
+  // --- 1. greedy.  Lane t builds the mask of earlier candidates s whose
+  // |cos(s, t)| >= delta; candidate i is accepted when its mask misses every
+  // accepted candidate (max |cos| < delta) and fewer than k_max are accepted.
   unsigned long long acc = 1ull;  // bit i: candidate i accepted
   int nsel = 1;
   if (nc > 1) {
-    double gc[KMAX];
-#pragma unroll
-    for (int i = 0; i < KMAX; i++) gc[i] = (t < nc) ? G[t + i * KMAX] : 0.0;  // G symmetric
     const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
-    const double r0 = rdlane(rt, 0);
-    double run = (t < nc) ? fabs(gc[0] * r0 * rt) : 0.0;
-#pragma unroll
-    for (int i = 1; i < KMAX; i++) {
-      if (i < nc) {
-        const double mi = rdlane(run, i);
-        if (mi < delta && nsel < kmax) {
-          acc |= 1ull << i;
-          nsel++;
-          const double ri = rdlane(rt, i);
-          if (t > i && t < nc) run = fmax(run, fabs(gc[i] * ri * rt));
-        }
+    s_r[t] = rt;
+    __syncthreads();
+    unsigned long long conf = 0ull;
+#pragma unroll 8
+    for (int q = 0; q < KMAX; q++) {
+      const double c = fabs(G[t + q * KMAX] * s_r[q] * rt);  // G symmetric: G(q, t)
+      conf |= (q < t && q < nc && c >= delta) ? (1ull << q) : 0ull;
+    }
+    for (int i = 1; i < nc; i++) {
+      const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(conf >> 32), i);
+      const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)conf, i);
+      const unsigned long long ci = ((unsigned long long)hi << 32) | lo;
+      if ((ci & acc) == 0ull && nsel < kmax) {
+        acc |= 1ull << i;
+        nsel++;
       }
     }
   }
@@ -461,76 +477,75 @@ __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, dou
     st->selpos[spos] = t;
   }
   __syncthreads();
-  // --- 2. permute_marked's walk
-  int mp = (t < fjb) ? s_sel[t] : -1;  // position of mark t
-  int tp0 = -1, tc0 = 0, tp1 = -1, tc1 = 0;
-  int ntouch = 0, nswap = 0;
-  auto marked = [&](int p) -> bool { return __ballot(mp == p) != 0ull; };
-  auto entry = [&](int p) -> int {  // index of p in the touched map (added if absent)
-    const unsigned long long b0 = __ballot(tp0 == p);
-    if (b0) return __ffsll((long long)b0) - 1;
-    const unsigned long long b1 = __ballot(tp1 == p);
-    if (b1) return 64 + __ffsll((long long)b1) - 1;
-    const int e = ntouch++;
-    tp0 = (t == e) ? p : tp0;
-    tc0 = (t == e) ? p : tc0;
-    tp1 = (t + 64 == e) ? p : tp1;
-    tc1 = (t + 64 == e) ? p : tc1;
-    return e;
-  };
-  auto content = [&](int e) -> int {
-    const int x0 = __shfl(tc0, e & 63, 64), x1 = __shfl(tc1, e & 63, 64);
-    return e < 64 ? x0 : x1;
-  };
-  auto set_content = [&](int e, int v) {
-    tc0 = (t == e) ? v : tc0;
-    tc1 = (t + 64 == e) ? v : tc1;
-  };
-  auto swp = [&](int p, int q) {
-    mp = (mp == p) ? q : ((mp == q) ? p : mp);
-    const int ep = entry(p), eq = entry(q);
-    const int cp = content(ep), cq = content(eq);
-    set_content(ep, cq);
-    set_content(eq, cp);
-    nswap++;
-  };
-  int jb = 0, jt = nsub - 1;
-  for (int sidx = 0; sidx < fjb; sidx++) {
-    const int jc = s_sel[sidx];
-    while (jb < jt && marked(jt)) {
-      swp(jt, jb);
-      while (jb < nsub && marked(jb)) jb++;
-    }
-    if (marked(jc)) {
-      while (jb < nsub && marked(jb)) jb++;
-      if (jc <= jb || jc < fjb) continue;
-      if (!marked(jb)) {
-        swp(jc, jb);
-        jb++;
-      }
-    }
-  }
-  // --- 3. plan
-  const bool mv0 = t < ntouch && tp0 != tc0;
-  const bool mv1 = t + 64 < ntouch && tp1 != tc1;
-  const unsigned long long b0 = __ballot(mv0), b1 = __ballot(mv1);
+  // --- 2. permute_marked's walk, in closed form.  With the accepted
+  // positions S (|S| = fjb), jt = n_sub - 1 and the free front slots
+  // F = [0, fjb) \ S in increasing order, the walk (host permute_walk)
+  // amounts to:
+  //  - when jt is in S (and jt > 0), its first pass swaps jt with position 0:
+  //    if jt >= fjb the column A at jt goes to 0 and consumes F[0]; the one at
+  //    0, if accepted (B), moves on to F[0]; the unmarked column from F[0]
+  //    goes to jt.  If jt < fjb every column is accepted and A, B trade places;
+  //  - every other accepted column beyond the front (position >= fjb) takes
+  //    the next free slot in acceptance order, the slot's unmarked column
+  //    going to its position; accepted columns in the front stay.
+  // (Checked against the host walk exhaustively for n_sub <= 7 and on random
+  // selections up to 300 columns.)  Lane t handles acceptance rank t.
+  __shared__ int s_occ[KMAX], s_slot[KMAX];
   const unsigned long long below = (1ull << t) - 1ull;
-  if (mv0) {
-    const int q = __popcll(b0 & below);
-    perm[2 * q] = j + tp0;
-    perm[2 * q + 1] = j + tc0;
+  const int jt = nsub - 1;
+  const int ps = (t < fjb) ? s_sel[t] : -1;
+  s_occ[t] = 0;
+  __syncthreads();
+  if (t < fjb && ps < fjb) s_occ[ps] = 1;
+  __syncthreads();
+  const unsigned long long freem = __ballot(t < fjb && s_occ[t] == 0);
+  if ((freem >> t) & 1ull) s_slot[__popcll(freem & below)] = t;
+  const bool zero_sel = s_occ[0] != 0;
+  const bool hasA = jt > 0 && __ballot(t < fjb && ps == jt) != 0ull;
+  const bool farA = hasA && jt >= fjb;
+  const unsigned long long farm = __ballot(t < fjb && ps >= fjb && !(hasA && ps == jt));
+  __syncthreads();
+  int mp = ps, um_dst = 0, um_src = 0;
+  bool um = false;
+  if (t < fjb) {
+    if (hasA && ps == jt) {  // A
+      mp = 0;
+      if (farA) {
+        um = true;
+        um_dst = jt;
+        um_src = s_slot[0];
+      }
+    } else if (ps >= fjb) {
+      const int k = __popcll(farm & below) + (farA ? 1 : 0);
+      mp = s_slot[k];
+      um = true;
+      um_dst = ps;
+      um_src = mp;
+    } else if (hasA && ps == 0) {  // B
+      mp = farA ? s_slot[0] : jt;
+    }
   }
-  if (mv1) {
-    const int q = __popcll(b0) + __popcll(b1 & below);
-    perm[2 * q] = j + tp1;
-    perm[2 * q + 1] = j + tc1;
+  const int nswap = __popcll(farm) + (farA ? (zero_sel ? 2 : 1) : (hasA ? 1 : 0));
+  // --- 3. plan: a (dst, src) pair for every accepted column and every
+  // unmarked column that moved; the candidate rank of each panel column; the
+  // panel form.
+  const bool mvm = t < fjb && mp != ps;
+  const unsigned long long bm = __ballot(mvm), bu = __ballot(um);
+  if (mvm) {
+    const int q = __popcll(bm & below);
+    perm[2 * q] = j + mp;
+    perm[2 * q + 1] = j + ps;
   }
-  // every accepted column now sits in [0, fjb); mark t rides with the
-  // column of acceptance rank t, candidate rank selpos[t]
+  if (um) {
+    const int q = __popcll(bm) + __popcll(bu & below);
+    perm[2 * q] = j + um_dst;
+    perm[2 * q + 1] = j + um_src;
+  }
+  // every accepted column now sits in [0, fjb)
   if (t < fjb) perm[4 * KMAX + mp] = s_rank[t];
   if (t == 0) {
     st->fjb = fjb;
-    st->npair = __popcll(b0) + __popcll(b1);
+    st->npair = __popcll(bm) + __popcll(bu);
     st->nswap = nswap;
     st->fast = (fjb >= 2 && m - j >= fast_rows) ? 1 : 0;
   }

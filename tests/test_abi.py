@@ -112,3 +112,73 @@ def test_qp3_forward_is_host_lapack():
     qr, jpv, tau, work, info = lapack.dgeqp3(a.copy(order="F"))
     assert np.array_equal(jp, jpv)
     assert np.allclose(b, qr, rtol=0, atol=1e-13) and np.allclose(t, tau, atol=1e-15)
+
+
+def _closed_form_arrangement(ncols, sel):
+    """The placement permute_marked produces, in the closed form the device
+    kernel k_select computes (kernels.hip): final content of each position
+    and the number of swaps."""
+    fjb, jt = len(sel), ncols - 1
+    S = set(sel)
+    free = [q for q in range(fjb) if q not in S]
+    has_a = jt in S and jt > 0
+    far_a = has_a and jt >= fjb
+    content = list(range(ncols))
+    k, nswap = (1 if far_a else 0), 0
+    for s in sel:
+        if has_a and s == jt:
+            continue
+        if s >= fjb:
+            f = free[k]
+            k += 1
+            content[f], content[s] = s, f
+            nswap += 1
+    if far_a:
+        f0 = free[0]
+        if 0 in S:
+            content[0], content[f0], content[jt] = jt, 0, f0
+            nswap += 2
+        else:
+            content[0], content[jt] = jt, 0
+            nswap += 1
+    elif has_a:
+        content[0], content[jt] = jt, 0
+        nswap += 1
+    return content, nswap
+
+
+def _walk_arrangement(ncols, sel):
+    import qrdm_amd
+    sw = np.zeros(2 * ncols + 4, np.int32)
+    s = np.array(sel, np.int32)
+    k = qrdm_amd.lib().qrdm_permute_walk(ncols, s.ctypes.data_as(ip), len(sel),
+                                         sw.ctypes.data_as(ip))
+    content = list(range(ncols))
+    for i in range(k):
+        a, b = sw[2 * i], sw[2 * i + 1]
+        content[a], content[b] = content[b], content[a]
+    return content, k
+
+
+def test_permute_closed_form_matches_walk():
+    """The device's closed-form placement equals the swap walk: exhaustively
+    for up to 6 columns, and on random selections of up to 64 of 300."""
+    import itertools
+    for ncols in range(1, 7):
+        for fjb in range(1, ncols + 1):
+            for sel in itertools.permutations(range(ncols), fjb):
+                assert _closed_form_arrangement(ncols, list(sel)) == \
+                    _walk_arrangement(ncols, list(sel)), (ncols, sel)
+    rng = np.random.default_rng(11)
+    for _ in range(3000):
+        ncols = int(rng.integers(1, 300))
+        fjb = int(rng.integers(1, min(64, ncols) + 1))
+        sel = [int(x) for x in rng.choice(ncols, fjb, replace=False)]
+        if rng.random() < 0.4 and ncols - 1 not in sel:
+            sel[int(rng.integers(fjb))] = ncols - 1
+        if rng.random() < 0.4 and 0 not in sel:
+            sel[int(rng.integers(fjb))] = 0
+        if len(set(sel)) != fjb:
+            continue
+        assert _closed_form_arrangement(ncols, sel) == _walk_arrangement(ncols, sel), \
+            (ncols, sel)
