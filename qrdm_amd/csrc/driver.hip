@@ -372,10 +372,8 @@ int run_dm(const Args& x) {
           launch_panel_fast(A, lda, j0, m, tau_dm, x.tau, B.G, B.perm + kRankOff, B.gram_part,
                             B.R1, B.R1inv, B.G2, B.T, B.M, B.V, B.ldv, B.st, s);
         launch_panel_exact(A, lda, j0, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
-        launch_build_v(A, lda, j0, m, B.ldv, B.st, B.V, B.ldv, s, RUN_EXACT);
-        launch_gram(B.V, B.ldv, j0, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s, B.st,
-                    RUN_EXACT);
-        launch_larft(B.S, x.tau, B.st, B.T, s, RUN_EXACT);
+        launch_vgram_exact(A, lda, j0, m, B.st, B.V, B.ldv, B.gram_part, s);
+        launch_larft(B.gram_part, x.tau, B.st, B.T, s, RUN_EXACT, vgram_exact_slabs(j0, m));
         g_prof.end(pq, s, 0.0);
         // --- dlarfb (src/dgeqrdm_work.c:762-767) from c0 (set by the panel)
         int pl = g_prof.begin(PC_LARFB, s);

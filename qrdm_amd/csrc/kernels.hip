@@ -336,6 +336,40 @@ void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, d
 }
 
 // ---------------------------------------------------------------------------
+// out = X^T X (64 x 64, column-major) for the 64-row slab X held in LDS as
+// X[c * (GRAM_ROWS + 1) + r]; 256 threads, wave w computes rows 16w..16w+15
+// on the matrix cores.
+__device__ __forceinline__ void gram_slab_acc(const double* X, dbl4 (&acc)[4]) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, li = lane & 15;
+#pragma unroll 4
+  for (int k0 = 0; k0 < GRAM_ROWS; k0 += 4) {
+    const double a = X[(16 * wv + li) * (GRAM_ROWS + 1) + k0 + g];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+          a, X[(16 * u + li) * (GRAM_ROWS + 1) + k0 + g], acc[u], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void gram_store(const dbl4 (&acc)[4], double* out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane >> 4, li = lane & 15;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) out[(16 * wv + g + 4 * v) + (16 * u + li) * KMAX] = acc[u][v];
+}
+
+__device__ __forceinline__ void gram_slab_mfma(const double* X, double* out) {
+  dbl4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  gram_slab_acc(X, acc);
+  gram_store(acc, out);
+}
+
+// ---------------------------------------------------------------------------
 // Gram matrix of up to 64 columns over a row range: partial sums per
 // GRAM_ROWS-row slab, then a fixed-order reduction.  Used for the candidate
 // cosines (cblas_dsyrk, src/dgeqrdm_work.c:376-379) and for V^T V (dlarft).
@@ -373,25 +407,7 @@ __global__ __launch_bounds__(256) void k_gram_partial(const double* __restrict__
     X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
   }
   __syncthreads();
-  const int p0 = (threadIdx.x >> 4) * 4, q0 = (threadIdx.x & 15) * 4;
-  double acc[4][4] = {};
-  for (int r = 0; r < GRAM_ROWS; r++) {
-    double xp[4], xq[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-      xp[a] = X[(p0 + a) * (GRAM_ROWS + 1) + r];
-      xq[a] = X[(q0 + a) * (GRAM_ROWS + 1) + r];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-      for (int b = 0; b < 4; b++) acc[a][b] = fma(xp[a], xq[b], acc[a][b]);
-  }
-  double* out = partial + (size_t)blockIdx.x * KMAX * KMAX;
-#pragma unroll
-  for (int a = 0; a < 4; a++)
-#pragma unroll
-    for (int b = 0; b < 4; b++) out[(p0 + a) + (q0 + b) * KMAX] = acc[a][b];
+  gram_slab_mfma(X, partial + (size_t)blockIdx.x * KMAX * KMAX);
 }
 
 __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ partial,
@@ -400,9 +416,23 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* __restrict__ 
   __shared__ double part[4][64];
   if (!step_runs(st, mode)) return;
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  // entries below the diagonal take the value of their mirror: G is exactly
+  // symmetric, the upper triangle as dsyrk('U') forms it
+  const int pr = e & 63, pc = e >> 6;
+  const int es = (pr > pc) ? pc + pr * KMAX : e;
   const int q = threadIdx.x >> 6;
+  // slabs q, q+4, ... summed eight loads at a time (index clamped, surplus masked)
   double s = 0.0;
-  for (int b = q; b < nslab; b += 4) s += partial[(size_t)b * KMAX * KMAX + e];
+  for (int b0 = q; b0 < nslab; b0 += 32) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int b = b0 + 4 * u;
+      v[u] = partial[(size_t)(b < nslab ? b : q) * KMAX * KMAX + es];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += (b0 + 4 * u < nslab) ? v[u] : 0.0;
+  }
   part[q][threadIdx.x & 63] = s;
   __syncthreads();
   if (q == 0) G[e] = (part[0][threadIdx.x] + part[1][threadIdx.x]) +
@@ -726,39 +756,119 @@ void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevS
 }
 
 // ---------------------------------------------------------------------------
+// The DM loop's exact-panel V (RUN_EXACT): each workgroup builds the V rows
+// [rs, rs + EX_ROWS) from the panel (unit lower trapezoidal, fjb_cmp
+// columns, zero padded) in 64-row slabs, writes them to V and their Gram
+// V^T V to partial[blockIdx.x] (zero past m).  Workgroup 0 also zeroes the V
+// rows [j - 64, j) above the panel, which the trailing update relies on.
+constexpr int EX_ROWS = 4 * GRAM_ROWS;
+
+__global__ __launch_bounds__(256) void k_vgram_exact(const double* __restrict__ A, int lda,
+                                                     int m, const DevState* st, double* V,
+                                                     int ldv, double* partial) {
+  __shared__ double X[KMAX * (GRAM_ROWS + 1)];
+  if (!step_runs(st, RUN_EXACT)) return;
+  const int j = st->j, kk = st->fjb_cmp;
+  const int tid = threadIdx.x;
+  dbl4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int sub = 0; sub < EX_ROWS / GRAM_ROWS; sub++) {
+    const int rs = j + blockIdx.x * EX_ROWS + sub * GRAM_ROWS;
+    if (rs >= m) break;
+    double xv[KMAX * GRAM_ROWS / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * GRAM_ROWS / 256; q++) {
+      const int e = tid + 256 * q, c = e / GRAM_ROWS, r = rs + e % GRAM_ROWS;
+      const int rr = r - j;
+      xv[q] = (c < kk && r < m && rr >= c) ? (rr == c ? 1.0 : A[(size_t)(j + c) * lda + r]) : 0.0;
+    }
+    __syncthreads();  // the previous slab's products are done with X
+#pragma unroll
+    for (int q = 0; q < KMAX * GRAM_ROWS / 256; q++) {
+      const int e = tid + 256 * q, c = e / GRAM_ROWS, r = e % GRAM_ROWS;
+      X[c * (GRAM_ROWS + 1) + r] = xv[q];
+      if (rs + r < m) V[(size_t)c * ldv + rs + r] = xv[q];
+    }
+    __syncthreads();
+    gram_slab_acc(X, acc);
+  }
+  if (blockIdx.x == 0) {
+    const int z0 = j >= KMAX ? j - KMAX : 0;
+    for (int e = tid; e < KMAX * KMAX; e += 256) {
+      const int r = z0 + (e & 63), c = e >> 6;
+      if (r < j) V[(size_t)c * ldv + r] = 0.0;
+    }
+  }
+  gram_store(acc, partial + (size_t)blockIdx.x * KMAX * KMAX);
+}
+
+int vgram_exact_slabs(int j0, int m) {
+  const int ns = (m - j0 + EX_ROWS - 1) / EX_ROWS;
+  return ns > 0 ? ns : 1;
+}
+
+void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st, double* V,
+                        int ldv, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_vgram_exact, dim3(vgram_exact_slabs(j0, m)), dim3(256), 0, s, A, lda, m,
+                     st, V, ldv, partial);
+}
+
+// ---------------------------------------------------------------------------
 // dlarft, DIRECT='F', STOREV='C' (call site src/dgeqrdm_work.c:751-754):
 // T(0:i,i) = -tau_i T(0:i,0:i) (V^T v_i)(0:i), T(i,i) = tau_i, from S = V^T V.
-__global__ __launch_bounds__(64) void k_larft(const double* __restrict__ S,
-                                              const double* __restrict__ tau,
-                                              const DevState* st, double* T, int mode) {
+// With nslab > 0, S is first summed from nslab partial Grams (the exact
+// panel's V slabs); 256 threads, the recurrence on the first 64.
+__global__ __launch_bounds__(256) void k_larft(const double* __restrict__ S,
+                                               const double* __restrict__ tau,
+                                               const DevState* st, double* T, int mode,
+                                               int nslab) {
+  __shared__ double Ss[KMAX * KMAX];
   __shared__ double Ts[KMAX * KMAX];
-  __shared__ double tmp[KMAX];
+  __shared__ double tmp[KMAX], ts[KMAX];
   if (!step_runs(st, mode)) return;
   if (mode != RUN_ALWAYS) tau += st->j;
-  const int p = threadIdx.x;
+  const int tid = threadIdx.x, p = tid;
   const int kk = st->fjb_cmp;
-  for (int c = 0; c < KMAX; c++) Ts[p + c * KMAX] = 0.0;
+  for (int e = tid; e < KMAX * KMAX; e += 256) {
+    double v = 0.0;
+    if (nslab > 0) {
+      for (int b0 = 0; b0 < nslab; b0 += 8) {  // eight loads in flight
+        double pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          pv[u] = S[(size_t)(b0 + u < nslab ? b0 + u : 0) * KMAX * KMAX + e];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v += (b0 + u < nslab) ? pv[u] : 0.0;
+      }
+    } else {
+      v = S[e];
+    }
+    Ss[e] = v;
+    Ts[e] = 0.0;
+  }
+  if (tid < KMAX) ts[tid] = tid < kk ? tau[tid] : 0.0;
   __syncthreads();
   for (int i = 0; i < kk; i++) {
-    const double ti = tau[i];
+    const double ti = ts[i];
     if (ti != 0.0) {
-      tmp[p] = (p < i) ? -ti * S[p + i * KMAX] : 0.0;
+      if (p < KMAX) tmp[p] = (p < i) ? -ti * Ss[p + i * KMAX] : 0.0;
       __syncthreads();
-      double s = 0.0;
+      double sacc = 0.0;
       if (p < i)
-        for (int q = p; q < i; q++) s += Ts[p + q * KMAX] * tmp[q];
+        for (int q = p; q < i; q++) sacc += Ts[p + q * KMAX] * tmp[q];
       __syncthreads();
-      if (p < i) Ts[p + i * KMAX] = s;
+      if (p < i) Ts[p + i * KMAX] = sacc;
       if (p == i) Ts[i + i * KMAX] = ti;
     }
     __syncthreads();
   }
-  for (int c = 0; c < KMAX; c++) T[p + c * KMAX] = Ts[p + c * KMAX];
+  for (int e = tid; e < KMAX * KMAX; e += 256) T[e] = Ts[e];
 }
 
 void launch_larft(const double* S, const double* tau, const DevState* st, double* T,
-                  hipStream_t s, int mode) {
-  hipLaunchKernelGGL(k_larft, dim3(1), dim3(64), 0, s, S, tau, st, T, mode);
+                  hipStream_t s, int mode, int nslab) {
+  hipLaunchKernelGGL(k_larft, dim3(1), dim3(256), 0, s, S, tau, st, T, mode, nslab);
 }
 
 // ---------------------------------------------------------------------------

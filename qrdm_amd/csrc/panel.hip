@@ -505,13 +505,14 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
 // panel (R above and on the diagonal, V1 below), tau, T and M; or only the
 // fallback flag.  st->fjb_cmp = k.
 __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
-                                               const double* __restrict__ R1, double* A,
+                                               const double* __restrict__ R1,
+                                               const double* __restrict__ R1inv, double* A,
                                                int lda, double tau_dm, double* tau,
                                                double* T_out, double* M_out, DevState* st) {
   __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
   __shared__ double SA[KMAX][LDP];  // R^-1
-  __shared__ double SW[KMAX][LDP];  // P_top, then Q11 and its LU, then L^-T
-  __shared__ double SX[KMAX][LDP];  // G2 and R2, then U
+  __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
+  __shared__ double SX[KMAX][LDP];  // G2, F or R2, R1^-1, then U
   __shared__ double Y[3][BLK][BLK + 1];
   __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
   __shared__ int s_done, s_why;
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
   const int tid = threadIdx.x;
   const int k1 = st->k1;
   {
-    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256], gw[KMAX * KMAX / 256];
+    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256];
 #pragma unroll
     for (int q = 0; q < KMAX * KMAX / 256; q++) {
       // e walks rows fastest: coalesced column-major reads
@@ -529,42 +530,127 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
       const bool in = r < k1 && c < k1;
       gx[q] = in ? G2[r + c * KMAX] : (r == c ? 1.0 : 0.0);
       gr[q] = R1[r + c * KMAX];
-      gw[q] = in ? A[(size_t)(j + c) * lda + j + r] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < KMAX * KMAX / 256; q++) {
       const int e = tid + 256 * q, r = e & 63, c = e >> 6;
       SX[r][c] = gx[q];
       SR[r][c] = gr[q];
-      SW[r][c] = gw[q];
     }
   }
   __syncthreads();
-  // 1. R2 = chol(G2); G2 is close to I after the first pass
-  int why = 0;
-  chol_blocked(
-      SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv, &s_done,
-      &s_why, &why);
-  if (why) {
-    if (tid == 0) st->flags |= 2;
-    return;
+  // 1-3. R = R2 R1 with R2^T R2 = G2 and R^-1 (into SR and SA).  After the
+  // first pass G2 = I + E with E tiny: then R2 = I + F, F upper triangular
+  // with F + F^T + F^T F = E, is the fixed point of F = triu_half(E - F^T F)
+  // (triu_half: the upper triangle with the diagonal halved), reached from
+  // F = triu_half(E) with an error of order |E|^(k+2) after k more passes,
+  // and R2^-1 = I - F + F^2 - ...; each pass is one 64 x 64 x 64 product on
+  // the matrix cores.  Larger E: blocked Cholesky and inverse.
+  double emax;
+  {
+    double em = 0.0;
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      em = fmax(em, fabs(SX[r][c] - (r == c ? 1.0 : 0.0)));
+    }
+    em = fmax(em, __shfl_xor(em, 32, 64));
+    em = fmax(em, __shfl_xor(em, 16, 64));
+    em = fmax(em, __shfl_xor(em, 8, 64));
+    em = fmax(em, __shfl_xor(em, 4, 64));
+    em = fmax(em, __shfl_xor(em, 2, 64));
+    em = fmax(em, __shfl_xor(em, 1, 64));
+    if ((tid & 63) == 0) rinv[tid >> 6] = em;
+    __syncthreads();
+    emax = fmax(fmax(rinv[0], rinv[1]), fmax(rinv[2], rinv[3]));
+    __syncthreads();
   }
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
-    if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
-  }
-  __syncthreads();
-  // 2. R = R2 R1 (identity-padded for the inverse)
   dbl4p acc[4];
-  mfma_64(acc, SX, false, SR);
-  __syncthreads();
-  mfma_store(SR, acc);
-  __syncthreads();
-  if (tid >= k1 && tid < KMAX) SR[tid][tid] = 1.0;
-  __syncthreads();
-  // 3. R^-1 -> SA
-  tri_inv_upper(SR, false, SA, rinv, Y);
+  int why = 0;
+  if (emax < 1e-3) {
+    // passes: |E|^(npass + 2) <= 2^-60 (relative to the identity)
+    int npass = 0;
+    for (double err = emax * emax; err > 8.7e-19 && npass < 6; err *= emax) npass++;
+    // SW = E, SX = F = triu_half(E)
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      const double ev = SX[r][c] - (r == c ? 1.0 : 0.0);
+      SW[r][c] = ev;
+      SX[r][c] = (r < c) ? ev : (r == c ? 0.5 * ev : 0.0);
+    }
+    __syncthreads();
+    for (int ps = 0; ps < npass; ps++) {
+      mfma_64(acc, SX, true, SX);  // F^T F
+      __syncthreads();
+      const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
+          const double x = SW[r][c] - acc[u][v];
+          SX[r][c] = (r < c) ? x : (r == c ? 0.5 * x : 0.0);
+        }
+      __syncthreads();
+    }
+    // R = R1 + F R1
+    mfma_64(acc, SX, false, SR);
+    __syncthreads();
+    {
+      const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) SR[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] += acc[u][v];
+    }
+    // R2^-1 = I - F (I - F (...)) into SW
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      SW[r][c] = (r == c ? 1.0 : 0.0) - SX[r][c];
+    }
+    __syncthreads();
+    for (int ps = 0; ps < npass; ps++) {
+      mfma_64(acc, SX, false, SW);
+      __syncthreads();
+      const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
+          SW[r][c] = (r == c ? 1.0 : 0.0) - acc[u][v];
+        }
+      __syncthreads();
+    }
+    // R^-1 = R1^-1 R2^-1
+    load_mat(SX, R1inv);
+    __syncthreads();
+    mfma_64(acc, SX, false, SW);
+    mfma_store(SA, acc);
+    __syncthreads();
+  } else {
+    chol_blocked(
+        SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv,
+        &s_done, &s_why, &why);
+    if (why) {
+      if (tid == 0) st->flags |= 2;
+      return;
+    }
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e >> 6, c = e & 63;
+      if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
+    }
+    __syncthreads();
+    mfma_64(acc, SX, false, SR);  // R = R2 R1, identity-padded for the inverse
+    __syncthreads();
+    mfma_store(SR, acc);
+    __syncthreads();
+    if (tid >= k1 && tid < KMAX) SR[tid][tid] = 1.0;
+    __syncthreads();
+    tri_inv_upper(SR, false, SA, rinv, Y);
+  }
   // 4. Q11 = P_top R^-1 -> SW
+  load_slab(SW, A, lda, j, j, j + k1, k1);
+  __syncthreads();
   mfma_64(acc, SW, false, SA);
   __syncthreads();
   mfma_store(SW, acc);
@@ -699,7 +785,8 @@ void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double*
   const int nslab = (m - j0 + PR_ROWS - 1) / PR_ROWS;
   hipLaunchKernelGGL(k_qgram, dim3(nslab), dim3(256), 0, s, A, lda, m, R1inv, st, gram_part);
   launch_gram_reduce(gram_part, nslab, G2, s, st, RUN_FAST);
-  hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, G2, R1, A, lda, tau_dm, tau, T, M, st);
+  hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, G2, R1, R1inv, A, lda, tau_dm, tau, T, M,
+                     st);
   hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, m, M, st, V, ldv);
 }
 

@@ -109,8 +109,14 @@ void launch_panel_exact(double* A, int lda, int j, int m, int fjb, double tau_dm
                         int mode = RUN_ALWAYS);
 void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevState* st,
                     double* V, int ldv, hipStream_t s, int mode = RUN_ALWAYS);
+// T from S = V^T V; with nslab > 0, S is summed from that many partial Grams.
 void launch_larft(const double* S, const double* tau, const DevState* st, double* T,
-                  hipStream_t s, int mode = RUN_ALWAYS);
+                  hipStream_t s, int mode = RUN_ALWAYS, int nslab = 0);
+// The DM loop's exact-panel V and its partial Grams (RUN_EXACT; j0 <= j
+// sizes the grid: vgram_exact_slabs(j0, m) partials).
+int vgram_exact_slabs(int j0, int m);
+void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st, double* V,
+                        int ldv, double* partial, hipStream_t s);
 // Block reflector from the left on rows [j, m), columns [c0, n):
 // trans = 1: C := (I - V T V^T)^T C (dgeqrdm's trailing update);
 // trans = 0: C := (I - V T V^T) C.
