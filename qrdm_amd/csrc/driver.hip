@@ -120,9 +120,8 @@ struct Buffers {
   DevState* st;
   double* work;      // n: partial norms (permuted with the columns)
   double* workold;   // n: norms at last recompute (positional)
-  double* topk_val;  // 2 * half
+  double* topk_val;   // topk_blocks(n) * 64: the top-k stage's per-workgroup lists
   int* topk_idx;
-  int topk_half;
   double* gram_part;  // nslab * 64 * 64
   double* G;          // 64 * 64
   double* S;          // 64 * 64 (V^T V)
@@ -138,7 +137,7 @@ struct Buffers {
   int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
   int* jp;     // n: jpvt on the device during the DM loop
   int* ncols;  // n: fjb_cmp of each DM step
-  unsigned* tickets;  // ceil(n/64) counters of the trailing update (zero between uses)
+  unsigned* tickets;  // counters (zero between uses); tickets[0]: the top-k stage
   int nsplit;
 };
 
@@ -146,15 +145,13 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   char* p = base;
   const int mpad = ((m + 63) / 64) * 64;
   const int nslab = (m + 31) / 32 + 1;  // slabs of the Gram and fast-panel passes
-  const int half = ((n + TOPK_CHUNK - 1) / TOPK_CHUNK) * KMAX + KMAX;
   const int nsplit = 16;  // row slabs of the trailing update's first product
   Buffers t;
   t.st = carve<DevState>(p, 1);
   t.work = carve<double>(p, n);
   t.workold = carve<double>(p, n);
-  t.topk_half = half;
-  t.topk_val = carve<double>(p, 2 * (size_t)half);
-  t.topk_idx = carve<int>(p, 2 * (size_t)half);
+  t.topk_val = carve<double>(p, (size_t)topk_blocks(n) * KMAX);
+  t.topk_idx = carve<int>(p, (size_t)topk_blocks(n) * KMAX);
   t.gram_part = carve<double>(p, (size_t)nslab * KMAX * KMAX);
   t.G = carve<double>(p, KMAX * KMAX);
   t.S = carve<double>(p, KMAX * KMAX);
@@ -353,7 +350,8 @@ int run_dm(const Args& x) {
       for (int b = 0; b < batch; b++) {
         // --- DM_perm (src/dgeqrdm_work.c:280-418) -------------------------
         int ps = g_prof.begin(PC_SELECT, s);
-        launch_topk(B.work, j0, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.topk_half, B.st, s);
+        launch_topk(B.work, j0, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.tickets, B.st,
+                    B.ncols, enable_stop, eta, s);
         // candidate Gram over rows j..m (cosines need it only when nc > 1;
         // the kernel reads nc from the state and zero-fills beyond it)
         launch_gram(A, lda, j0, m, B.st->cand_idx, KMAX, &B.st->nc, B.gram_part, B.G, s, B.st,
@@ -386,9 +384,9 @@ int run_dm(const Args& x) {
         // --- norm_update (src/dgeqrdm_work.c:774-778), stop test, next j ---
         int pn = g_prof.begin(PC_NORM, s);
         launch_norm_update(A, lda, j0, m, n, B.work, B.workold, B.st, tol3z, s);
-        launch_step_end(B.st, B.ncols, m, n, enable_stop, eta, s);
         g_prof.end(pn, s, 0.0);
       }
+      launch_step_end(B.st, B.ncols, m, n, enable_stop, eta, s);
       HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (x.trace && hst->it > it && hst->it - 1 < x.ntrace) {

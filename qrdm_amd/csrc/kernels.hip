@@ -195,144 +195,268 @@ __device__ __forceinline__ bool key_before(double va, int ia, double vb, int ib)
   return va > vb || (va == vb && ia < ib);
 }
 
-// Orders a wave's LDS accesses: its earlier stores are visible to its later
-// loads (one wave's LDS operations complete in order).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Bitonic sort of N (power of two) keys in LDS, descending in key_before
-// order, by N/2 threads (thread t compares one pair per stage).  A stage whose
-// compare distance jj is at most 64 keeps every pair of wave w inside the
-// keys [128w, 128w + 128), so it needs only the wave's own ordering; the
-// stages with jj >= 128 are separated by workgroup barriers.
-template <int N>
-__device__ void bitonic_lds(double* v, int* ix) {
-  const int t = threadIdx.x;
-  for (int k = 2; k <= N; k <<= 1) {
-    for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      const int lo = 2 * t - (t & (jj - 1));
-      const int hi = lo + jj;
-      const bool up = ((lo & k) == 0);
-      const double va = v[lo], vb = v[hi];
-      const int ia = ix[lo], ib = ix[hi];
-      const bool sw = up ? key_before(vb, ib, va, ia) : key_before(va, ia, vb, ib);
-      if (sw) {
-        v[lo] = vb;
-        v[hi] = va;
-        ix[lo] = ib;
-        ix[hi] = ia;
-      }
-      if (jj > 64 || (jj == 1 && k < N && (k << 1) > 128)) {
-        __syncthreads();
-      } else {
-        wave_lds_sync();
-      }
-    }
+// Value of x in lane (lane ^ JJ) of the wave, without LDS: DPP quad and row
+// permutations for JJ < 16, the gfx950 row-swap instructions above.
+template <int JJ>
+__device__ __forceinline__ int xor_lane(int x) {
+  const int lane = threadIdx.x & 63;
+  if constexpr (JJ == 1) {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (JJ == 2) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (JJ == 4) {
+    const int a = __builtin_amdgcn_update_dpp(0, x, 0x124, 0xF, 0xF, false);  // row_ror:4
+    const int b = __builtin_amdgcn_update_dpp(0, x, 0x12C, 0xF, 0xF, false);  // row_ror:12
+    return (lane & 4) ? a : b;
+  } else if constexpr (JJ == 8) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (JJ == 16) {
+    // rows of 16: the first result holds rows (0, 0, 2, 2), the second (1, 1, 3, 3)
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (lane & 16) ? (int)r[0] : (int)r[1];
+  } else {
+    static_assert(JJ == 32, "xor distances 1..32");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (lane & 32) ? (int)r[0] : (int)r[1];
   }
-  __syncthreads();
 }
 
-// Stage 1: each workgroup sorts TOPK_CHUNK keys and keeps its first KMAX.
-// in_idx == nullptr means the key index is its position.
-// The first level (in_idx == nullptr) reads work[j..n) with j from st.
-__global__ __launch_bounds__(512) void k_topk_chunk(const double* __restrict__ in_val,
-                                                    const int* __restrict__ in_idx, int n,
-                                                    double* out_val, int* out_idx,
-                                                    const DevState* st) {
-  __shared__ double v[TOPK_CHUNK];
-  __shared__ int ix[TOPK_CHUNK];
+template <int JJ>
+__device__ __forceinline__ double xor_lane(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = xor_lane<JJ>((int)b), hi = xor_lane<JJ>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One compare-exchange stage across lanes lane and lane ^ JJ: the lane keeps
+// the key that comes first (want_first) or second of its pair.  Keys are
+// distinct (padding keys carry distinct indices), so one comparison decides.
+template <int JJ>
+__device__ __forceinline__ void lane_cx(double& v, int& ix, bool want_first) {
+  const double pv = xor_lane<JJ>(v);
+  const int pi = xor_lane<JJ>(ix);
+  if (want_first == key_before(pv, pi, v, ix)) {
+    v = pv;
+    ix = pi;
+  }
+}
+
+// Sorts the wave's 64 keys (one per lane) into key_before order: the bitonic
+// network, blocks of K keys merged with distances K/2 .. 1.
+template <int K, int JJ>
+__device__ __forceinline__ void sort_stages(double& v, int& ix) {
+  const int lane = threadIdx.x & 63;
+  const bool first = (lane & JJ) == 0;       // lower position of the pair
+  const bool desc = (lane & (K & 63)) == 0;  // this K-block sorts descending
+  lane_cx<JJ>(v, ix, first == desc);
+  if constexpr (JJ > 1) sort_stages<K, JJ / 2>(v, ix);
+}
+
+template <int K = 2>
+__device__ __forceinline__ void wave_sort64(double& v, int& ix) {
+  sort_stages<K, K / 2>(v, ix);
+  if constexpr (K < 64) wave_sort64<2 * K>(v, ix);
+}
+
+// (v, ix) := the first 64 keys of two sorted 64-key lists (v, ix) and
+// (bv, bi), sorted: the lane-wise better of a[lane] and b[63 - lane] is a
+// bitonic sequence holding them, which six half-cleaner stages sort.
+__device__ __forceinline__ void wave_merge64(double& v, int& ix, double bv, int bi) {
+  const int lane = threadIdx.x & 63;
+  // b[63 - lane]: lane ^ 63 through the six distances
+  bv = xor_lane<32>(bv);
+  bi = xor_lane<32>(bi);
+  bv = xor_lane<16>(bv);
+  bi = xor_lane<16>(bi);
+  bv = xor_lane<8>(bv);
+  bi = xor_lane<8>(bi);
+  bv = xor_lane<4>(bv);
+  bi = xor_lane<4>(bi);
+  bv = xor_lane<2>(bv);
+  bi = xor_lane<2>(bi);
+  bv = xor_lane<1>(bv);
+  bi = xor_lane<1>(bi);
+  if (key_before(bv, bi, v, ix)) {
+    v = bv;
+    ix = bi;
+  }
+  lane_cx<32>(v, ix, (lane & 32) == 0);
+  lane_cx<16>(v, ix, (lane & 16) == 0);
+  lane_cx<8>(v, ix, (lane & 8) == 0);
+  lane_cx<4>(v, ix, (lane & 4) == 0);
+  lane_cx<2>(v, ix, (lane & 2) == 0);
+  lane_cx<1>(v, ix, (lane & 1) == 0);
+}
+
+// Merges the wave lists lv[0..nw) (sorted, 64 keys each) into lv[0] by a
+// tree over the waves; every wave of the workgroup calls it.
+template <int NW>
+__device__ __forceinline__ void block_merge_lists(double (*lv)[KMAX], int (*li)[KMAX], double& v,
+                                                  int& ix) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int h = NW / 2; h >= 1; h >>= 1) {
+    // wave w < h merges lists w and w + h into list w
+    if (wave < h) {
+      wave_merge64(v, ix, lv[wave + h][lane], li[wave + h][lane]);
+      lv[wave][lane] = v;
+      li[wave][lane] = ix;
+    }
+    __syncthreads();
+  }
+}
+
+// Whether the loop ends after the current step (src/dgeqrdm_work.c:782-785):
+// the last column reached, or the stop test met.
+__device__ __forceinline__ int step_done(const DevState* st, int m, int n, int enable_stop,
+                                         double eta) {
+  const int j = st->j, kk = st->fjb_cmp;
+  const double maxnrm = __longlong_as_double((long long)st->maxnrm_bits);
+  const int minmn = m < n ? m : n;
+  int done = (j + kk >= minmn) ? 1 : 0;
+  if (enable_stop && maxnrm * sqrt((double)(n - j - kk)) <= eta) done = 1;
+  return done;
+}
+
+// End of a DM step (run once, by one thread): ncols, the step record and
+// work, the stop test (enable_stop, eta absolute), the next j, done.
+__device__ void step_end_apply(DevState* st, int* ncols_dev, int m, int n, int enable_stop,
+                               double eta) {
+  st->pend = 0;
   if (st->done) return;
-  if (!in_idx) {
-    in_val += st->j;
-    n -= st->j;
+  const int j = st->j, kk = st->fjb_cmp, fjb = st->fjb;
+  double maxnrm;
+  {
+    const unsigned long long b = st->maxnrm_bits;
+    maxnrm = __longlong_as_double((long long)b);
   }
-  const int base = blockIdx.x * TOPK_CHUNK;
-  for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
-    int g = base + t;
-    if (g < n) {
-      v[t] = in_val[g];
-      ix[t] = in_idx ? in_idx[g] : g;
-    } else {
-      v[t] = -1.0;
-      ix[t] = 0x7fffffff;
-    }
-  }
-  __syncthreads();
-  bitonic_lds<TOPK_CHUNK>(v, ix);
-  for (int t = threadIdx.x; t < KMAX; t += blockDim.x) {
-    out_val[blockIdx.x * KMAX + t] = v[t];
-    out_idx[blockIdx.x * KMAX + t] = ix[t];
-  }
+  const int msub = m - j, nsub = n - j;
+  StepRec& r = st->last;
+  r.j = j;
+  r.kmax = st->kmax;
+  r.nc = st->nc;
+  r.fjb = fjb;
+  r.fjb_cmp = kk;
+  r.nswap = st->nswap;
+  r.panel = (st->fast && !(st->flags & 2)) ? 1 : 0;
+  r.maxnrm = maxnrm;
+  const double nct = (double)(n - st->c0);
+  StepWork& w = st->work;
+  w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
+  w.perm += 16.0 * (double)m * st->npair;
+  w.panel += 2.0 * msub * (double)fjb * fjb;
+  w.gemm1 += 2.0 * msub * nct * kk + 2.0 * kk * (double)kk * nct;
+  w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
+  w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
+  const int done = step_done(st, m, n, enable_stop, eta);  // from j before the advance
+  ncols_dev[st->it] = kk;
+  st->it += 1;
+  st->j = j + kk;
+  st->done = done;
+  st->maxnrm_bits = 0ull;
+  st->flags &= ~2;
 }
 
-// Stage 2: sort the <= TOPK_CHUNK survivors, keep kmax, count the candidates
-// (> tau * max, src/dgeqrdm_work.c:348-351).
-__global__ __launch_bounds__(512) void k_topk_final(const double* __restrict__ in_val,
-                                                    const int* __restrict__ in_idx, int n,
-                                                    int m_all, int n_all, int nb, double tau,
-                                                    DevState* st) {
-  __shared__ double v[TOPK_CHUNK];
-  __shared__ int ix[TOPK_CHUNK];
-  if (st->done) return;
-  const int j = st->j;
-  if (!in_idx) {
-    in_val += j;
-    n -= j;
-  }
-  // k_max = min(nb, m_sub, n_sub) (src/dgeqrdm_work.c:703), at most KMAX here
-  int kmax = nb < KMAX ? nb : KMAX;
-  if (m_all - j < kmax) kmax = m_all - j;
-  if (n_all - j < kmax) kmax = n_all - j;
-  for (int t = threadIdx.x; t < TOPK_CHUNK; t += blockDim.x) {
-    if (t < n) {
-      v[t] = in_val[t];
-      ix[t] = in_idx ? in_idx[t] : t;
-    } else {
-      v[t] = -1.0;
-      ix[t] = 0x7fffffff;
+// The step's candidates: the sorted first kmax keys of work[j..n) (value
+// descending, position ascending) and the candidate count (> tau * max,
+// src/dgeqrdm_work.c:348-351), after the previous step's end when that is
+// pending.  Each workgroup sorts TOPK_BLOCK keys (a 64-key list per wave,
+// merged over the waves) into its first 64, written to tmp; the last
+// workgroup to finish (atomic ticket) merges the lists and writes the
+// results and the pending step end.  Every workgroup reads the state before
+// taking its ticket and only the last one writes it.
+constexpr int TOPK_WAVES = 8;
+constexpr int TOPK_BLOCK = 64 * TOPK_WAVES;
+
+__global__ __launch_bounds__(64 * TOPK_WAVES) void k_topk(const double* __restrict__ work,
+                                                          int m_all, int n_all, int nb,
+                                                          double tau, double* tmp_val,
+                                                          int* tmp_idx, unsigned* ticket,
+                                                          DevState* st, int* ncols_dev,
+                                                          int enable_stop, double eta) {
+  __shared__ double lv[TOPK_WAVES][KMAX];
+  __shared__ int li[TOPK_WAVES][KMAX];
+  __shared__ int s_j, s_done, s_end, s_last;
+  if (threadIdx.x == 0) {
+    int j = st->j, done = st->done;
+    if (!done && st->pend) {  // the previous step's end, as step_end_apply computes it
+      done = step_done(st, m_all, n_all, enable_stop, eta);
+      j += st->fjb_cmp;
     }
+    s_j = j;
+    s_done = st->done;  // loop already over before this launch: nothing to do
+    s_end = done;       // the loop ends with the pending step: no sort
   }
   __syncthreads();
-  bitonic_lds<TOPK_CHUNK>(v, ix);
-  const int nk = n < kmax ? n : kmax;
-  if (threadIdx.x < KMAX) {
-    int t = threadIdx.x;
-    st->cand_val[t] = t < nk ? v[t] : 0.0;
-    st->cand_idx[t] = t < nk ? ix[t] : -1;
+  if (s_done) return;
+  const int j = s_j;
+  const bool sort = !s_end;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nsub = n_all - j;
+  const int nblk = gridDim.x;
+  double v = -1.0;
+  int ix = 0;
+  if (sort) {
+    const int t = blockIdx.x * TOPK_BLOCK + 64 * wave + lane;
+    // padding keys: -1 (norms are >= 0) with distinct indices past any column
+    v = (t < nsub) ? work[j + t] : -1.0;
+    ix = (t < nsub) ? t : 0x40000000 + t;
+    wave_sort64(v, ix);
+    lv[wave][lane] = v;
+    li[wave][lane] = ix;
+    __syncthreads();
+    block_merge_lists<TOPK_WAVES>(lv, li, v, ix);
+    if (wave == 0) {
+      tmp_val[blockIdx.x * KMAX + lane] = v;
+      tmp_idx[blockIdx.x * KMAX + lane] = ix;
+    }
+  }
+  __threadfence();
+  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == (unsigned)(nblk - 1)) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (sort) {
+    // wave w merges lists w, w + TOPK_WAVES, ... into its own
+    v = -1.0;  // a sorted list of padding keys
+    ix = 0x7fffffc0 + lane;
+    for (int b = wave; b < nblk; b += TOPK_WAVES)
+      wave_merge64(v, ix, tmp_val[b * KMAX + lane], tmp_idx[b * KMAX + lane]);
+    lv[wave][lane] = v;
+    li[wave][lane] = ix;
+    __syncthreads();
+    block_merge_lists<TOPK_WAVES>(lv, li, v, ix);
   }
   if (threadIdx.x == 0) {
-    double lim = tau * v[0];
-    int nc = 0;
-    while (nc < nk && v[nc] > lim) nc++;
-    st->nc = nc;
-    st->kmax = kmax;
+    if (st->pend) step_end_apply(st, ncols_dev, m_all, n_all, enable_stop, eta);
+    *ticket = 0u;
+  }
+  if (sort && wave == 0) {
+    // k_max = min(nb, m_sub, n_sub) (src/dgeqrdm_work.c:703), at most KMAX here
+    int kmax = nb < KMAX ? nb : KMAX;
+    if (m_all - j < kmax) kmax = m_all - j;
+    if (nsub < kmax) kmax = nsub;
+    const int nk = nsub < kmax ? nsub : kmax;
+    st->cand_val[lane] = lane < nk ? v : 0.0;
+    st->cand_idx[lane] = lane < nk ? ix : -1;
+    // the keys are sorted: those above tau * max form a prefix
+    const double lim = tau * __shfl(v, 0, 64);
+    const int nc = __popcll(__ballot(lane < nk && v > lim));
+    if (lane == 0) {
+      st->nc = nc;
+      st->kmax = kmax;
+    }
   }
 }
 
+int topk_blocks(int n) { return (n + TOPK_BLOCK - 1) / TOPK_BLOCK; }
+
 void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, double* tmp_val,
-                 int* tmp_idx, int half, DevState* st, hipStream_t s) {
-  // chunk structure for n - j0 >= n - j keys; surplus slots hold -1 and sort last
-  const double* cv = work;
-  const int* ci = nullptr;
-  int cn = n - j0;
-  double* ov = tmp_val;
-  int* oi = tmp_idx;
-  int cnt = n;  // count passed to the kernels (first level: n, shifted by j inside)
-  while (cn > TOPK_CHUNK) {
-    int nbk = (cn + TOPK_CHUNK - 1) / TOPK_CHUNK;
-    hipLaunchKernelGGL(k_topk_chunk, dim3(nbk), dim3(512), 0, s, cv, ci, cnt, ov, oi, st);
-    cv = ov;
-    ci = oi;
-    cn = nbk * KMAX;
-    cnt = cn;
-    // alternate between the two halves of the scratch lists
-    ov = (ov == tmp_val) ? tmp_val + half : tmp_val;
-    oi = (oi == tmp_idx) ? tmp_idx + half : tmp_idx;
-  }
-  hipLaunchKernelGGL(k_topk_final, dim3(1), dim3(512), 0, s, cv, ci, cnt, m, n, nb, tau, st);
+                 int* tmp_idx, unsigned* ticket, DevState* st, int* ncols_dev, int enable_stop,
+                 double eta, hipStream_t s) {
+  const int nblk = topk_blocks(n - j0);
+  hipLaunchKernelGGL(k_topk, dim3(nblk > 0 ? nblk : 1), dim3(64 * TOPK_WAVES), 0, s, work, m, n,
+                     nb, tau, tmp_val, tmp_idx, ticket, st, ncols_dev, enable_stop, eta);
 }
 
 // ---------------------------------------------------------------------------
@@ -1227,6 +1351,8 @@ __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ 
                                                      double tol3z) {
   __shared__ unsigned long long s_rows[4];
   if (st->done) return;
+  // the step's end is pending from here on (run by the next k_topk or k_step_end)
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->pend = 1;
   const int j = st->j;
   const int kk = st->fjb_cmp;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1278,42 +1404,11 @@ void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* 
 // End of a DM step (src/dgeqrdm_work.c:740, :774-785): ncols[it] = fjb_cmp,
 // the step's record and algorithmic work, j += fjb_cmp, the stop test
 // maxnrm * sqrt(n_sub - fjb_cmp) <= eta, and the loop's end at j >= min(m,n).
+// The pending end of the last queued step (the next step's k_topk runs it
+// otherwise).
 __global__ void k_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop,
                            double eta) {
-  if (st->done) return;
-  const int j = st->j, kk = st->fjb_cmp, fjb = st->fjb;
-  double maxnrm;
-  {
-    const unsigned long long b = st->maxnrm_bits;
-    maxnrm = __longlong_as_double((long long)b);
-  }
-  const int msub = m - j, nsub = n - j;
-  StepRec& r = st->last;
-  r.j = j;
-  r.kmax = st->kmax;
-  r.nc = st->nc;
-  r.fjb = fjb;
-  r.fjb_cmp = kk;
-  r.nswap = st->nswap;
-  r.panel = (st->fast && !(st->flags & 2)) ? 1 : 0;
-  r.maxnrm = maxnrm;
-  const double nct = (double)(n - st->c0);
-  StepWork& w = st->work;
-  w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
-  w.perm += 16.0 * (double)m * st->npair;
-  w.panel += 2.0 * msub * (double)fjb * fjb;
-  w.gemm1 += 2.0 * msub * nct * kk + 2.0 * kk * (double)kk * nct;
-  w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
-  w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
-  ncols_dev[st->it] = kk;
-  st->it += 1;
-  st->j = j + kk;
-  const int minmn = m < n ? m : n;
-  int done = (j + kk >= minmn) ? 1 : 0;
-  if (enable_stop && maxnrm * sqrt((double)(nsub - kk)) <= eta) done = 1;
-  st->done = done;
-  st->maxnrm_bits = 0ull;
-  st->flags &= ~2;
+  if (st->pend) step_end_apply(st, ncols_dev, m, n, enable_stop, eta);
 }
 
 void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
@@ -1322,6 +1417,7 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
 }
 
 __global__ void k_dm_begin(DevState* st, int j, int minmn) {
+  st->pend = 0;
   st->j = j;
   st->it = 0;
   st->done = (j >= minmn) ? 1 : 0;

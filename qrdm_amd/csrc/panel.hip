@@ -30,10 +30,13 @@
 //
 // The one-workgroup kernels (k_chol1, k_recon) keep 64 x 64 matrices in LDS
 // and work in 16-column blocks: wave 0 factors a diagonal block with one lane
-// per column, passing pivots and pivot rows between lanes with readlane (no
-// barrier inside the block); all threads then solve the block row or column
-// (one lane per row or column, independent), and the matrix cores update the
-// trailing matrix.  Three barriers per block instead of one per column.
+// per row, taking the pivot and the pivot row from lane i with readlane (no
+// barrier inside the block; each column's reciprocal or inverse square root
+// is a hardware estimate refined by Newton steps, and the column's results
+// are computed before its stop test, which only decides what is kept); all
+// threads then solve the block row or column (one lane per row or column,
+// independent), and the matrix cores update the trailing matrix.  Three
+// barriers per block instead of one per column.
 #include "qrdm_internal.h"
 
 #include <math.h>
@@ -46,6 +49,24 @@ constexpr int LDP = KMAX + 1;  // padded leading dimension of LDS matrices
 constexpr int BLK = 16;        // block width of the one-workgroup factorisations
 
 typedef double dbl4p __attribute__((ext_vector_type(4)));
+
+// 1/sqrt(d) for a positive normal d: the hardware estimate and two Newton
+// steps (about one ulp; short dependency chain for the factorisations' pivots).
+__device__ __forceinline__ double rsqrt_nr(double d) {
+  const double h = 0.5 * d;
+  double y = __builtin_amdgcn_rsq(d);
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// 1/x for x in [1, 2]: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_nr12(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
 
 // S[r][c] -= sum_{t < 16} A(r, t) B(t, c) for r, c in [lo, nlim), 16 x 16
 // tiles on the matrix cores (wave w takes tiles w, w + 4, ...).
@@ -84,45 +105,46 @@ __device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, 
 // rinv[i] = 1 / R(i, i).
 template <class StopFn>
 __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn stopfn, double* rinv,
-                           int* s_done, int* s_why, int* why) {
+                                            int* s_done, int* s_why, int* why) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int bs = 0; bs < nlim; bs += BLK) {
     if (wave == 0) {
-      // lane c holds column bs + c of the block
-      const int c = lane & 15;
-      double col[BLK];
+      // lane r (= lane % 16) holds row bs + r of the symmetric diagonal block;
+      // both triangles are updated, so R(i, r) = S(r, i) rs is the lane's own
+      // value and row i comes from lane i by readlane
+      const int r = lane & 15;
+      double w[BLK];
 #pragma unroll
-      for (int r = 0; r < BLK; r++) col[r] = S[bs + r][bs + c];
+      for (int c = 0; c < BLK; c++) w[c] = S[bs + r][bs + c];
+      double rsr = 0.0;  // 1 / sqrt(pivot) of this lane's row once eliminated
       bool act = true;
       int done = BLK, code = 0;
 #pragma unroll
       for (int i = 0; i < BLK; i++) {
-        if (act) {
-          const int gi = bs + i;
-          if (gi >= nlim) {
-            act = false;
-            done = i;
-          } else {
-            const double d = rdlane(col[i], i);
-            const int stc = stopfn(gi, d);
-            if (stc) {
-              act = false;
-              done = i;
-              code = stc;
-            } else {
-              const double rs = 1.0 / sqrt(d);
-              const double ri = col[i] * rs;  // R(i, c)
-              col[i] = ri;
-              if (lane == 0) rinv[gi] = rs;
+        const int gi = bs + i;
+        const double d = rdlane(w[i], i);
+        const int stc = (gi < nlim) ? stopfn(gi, d) : 0;
+        const double rs = rsqrt_nr(d);  // committed only when the pivot is accepted
+        const bool go = act && gi < nlim && stc == 0;
+        if (act && !go) {
+          done = i;
+          code = stc;
+        }
+        act = go;
+        if (go) {
+          if (lane == 0) rinv[gi] = rs;
+          rsr = (r == i) ? rs : rsr;
+          // S(r, c) -= R(i, r) R(i, c) = S(r, i) rs^2 S(i, c) for rows r > i
+          const double a = (r > i) ? w[i] * rs * rs : 0.0;
 #pragma unroll
-              for (int r = i + 1; r < BLK; r++) col[r] = fma(-rdlane(ri, r), ri, col[r]);
-            }
-          }
+          for (int c = i + 1; c < BLK; c++) w[c] = fma(-a, rdlane(w[c], i), w[c]);
         }
       }
+      // rows < done: R(r, c) = S(r, c) / sqrt(pivot) on and above the diagonal
       if (lane < BLK) {
 #pragma unroll
-        for (int r = 0; r < BLK; r++) S[bs + r][bs + c] = (r < done && c < r) ? 0.0 : col[r];
+        for (int c = 0; c < BLK; c++)
+          S[bs + r][bs + c] = (r < done) ? ((c < r) ? 0.0 : w[c] * rsr) : w[c];
       }
       if (lane == 0) {
         *s_done = done;
@@ -171,59 +193,60 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
 // pivots stop with code 2.  Returns the number of columns eliminated;
 // dsg[i] = d_i, tl[i] = 1 + |a_i| (= tau_i), scv[i] = the multiplier scale.
 __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, const double (*SR)[LDP],
-                                 double thres, double* dsg, double* tl, double* scv,
-                                 int* s_done, int* s_why, int* why) {
+                                                 double thres, double* dsg, double* tl,
+                                                 double* scv, int* s_done, int* s_why,
+                                                 int* why) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int bs = 0; bs < nlim; bs += BLK) {
     if (wave == 0) {
-      const int c = lane & 15;
-      double col[BLK];
+      // lane r (= lane % 16) holds row bs + r of the diagonal block; the pivot
+      // row comes from lane i by readlane
+      const int r = lane & 15;
+      double w[BLK];
 #pragma unroll
-      for (int r = 0; r < BLK; r++) col[r] = W[bs + r][bs + c];
+      for (int c = 0; c < BLK; c++) w[c] = W[bs + r][bs + c];
+      // ||x_i|| < thres  <=>  (1 - a^2) < (thres / R(i, i))^2: lane r holds the
+      // bound of column bs + r (R's diagonal is positive)
+      double rt2;
+      {
+        const double rt = thres / SR[bs + r][bs + r];
+        rt2 = rt * rt;
+      }
       bool act = true;
       int done = BLK, code = 0;
 #pragma unroll
       for (int i = 0; i < BLK; i++) {
-        if (act) {
-          const int gi = bs + i;
-          if (gi >= nlim) {
-            act = false;
-            done = i;
-          } else {
-            const double al = rdlane(col[i], i);
-            const double aa = fabs(al);
-            const double om = 1.0 - aa;  // 2 - tau
-            const double xnorm = SR[gi][gi] * sqrt(fmax(om, 0.0) * (1.0 + aa));
-            int stc = 0;
-            if (!(al == al) || !(xnorm == xnorm)) stc = 2;
-            else if (gi >= 1 && xnorm < thres) stc = 1;
-            else if (om < 1e-6) stc = 2;  // tau near 2: dlarfg's tau = 0 branch or lost digits
-            if (stc) {
-              act = false;
-              done = i;
-              code = stc;
-            } else {
-              const double di = (al >= 0.0) ? -1.0 : 1.0;
-              const double sc = di / (-1.0 - aa);
-              if (lane == 0) {
-                dsg[gi] = di;
-                tl[gi] = 1.0 + aa;
-                scv[gi] = sc;
-              }
-              const double wic = col[i];  // W(i, c)
-#pragma unroll
-              for (int r = i + 1; r < BLK; r++) {
-                const double lv = rdlane(col[r], i) * sc;  // L(r, i)
-                const double upd = fma(-lv, wic, col[r]);
-                col[r] = (c > i) ? upd : ((c == i) ? lv : col[r]);
-              }
-            }
+        const int gi = bs + i;
+        const double al = rdlane(w[i], i);
+        const double aa = fabs(al);
+        const double om = 1.0 - aa;  // 2 - tau
+        const double x2 = fmax(om, 0.0) * (1.0 + aa);  // (||x_i|| / R(i, i))^2
+        const double di = (al >= 0.0) ? -1.0 : 1.0;
+        const double sc = -di * rcp_nr12(1.0 + aa);  // d_i / (-1 - |a_i|)
+        const int stc = (gi >= nlim) ? 0 : (!(al == al) ? 2 : ((gi >= 1 && x2 < rdlane(rt2, i)) ? 1 : (
+                        // tau near 2: dlarfg's tau = 0 branch or lost digits
+                        (om < 1e-6) ? 2 : 0)));
+        const bool go = act && gi < nlim && stc == 0;
+        if (act && !go) {
+          done = i;
+          code = stc;
+        }
+        act = go;
+        if (go) {
+          if (lane == 0) {
+            dsg[gi] = di;
+            tl[gi] = 1.0 + aa;
+            scv[gi] = sc;
           }
+          const double lv = (r > i) ? w[i] * sc : 0.0;  // L(r, i)
+#pragma unroll
+          for (int c = i + 1; c < BLK; c++) w[c] = fma(-lv, rdlane(w[c], i), w[c]);
+          w[i] = (r > i) ? lv : w[i];
         }
       }
       if (lane < BLK) {
 #pragma unroll
-        for (int r = 0; r < BLK; r++) W[bs + r][bs + c] = col[r];
+        for (int c = 0; c < BLK; c++) W[bs + r][bs + c] = w[c];
       }
       if (lane == 0) {
         *s_done = done;

@@ -7,7 +7,6 @@
 namespace qrdm {
 
 constexpr int KMAX = 64;          // largest DM block the kernels handle
-constexpr int TOPK_CHUNK = 1024;  // keys sorted per workgroup by the top-k stage
 constexpr int GRAM_ROWS = 64;     // rows per workgroup of the Gram kernel
 
 // Record of one completed DM step (kept for tracing).
@@ -46,6 +45,7 @@ struct DevState {
   int c0;      // first column of the current step's trailing update
   int npair;   // (dst, src) column pairs of the current step's permutation
   int nswap;   // swaps made by the current step's permutation walk
+  int pend;    // 1: the current step's end (k_step_end's work) has not run yet
   StepRec last;
   StepWork work;
 };
@@ -56,6 +56,14 @@ __device__ __forceinline__ double rdlane(double x, int l) {
   const int lo = __builtin_amdgcn_readlane((int)b, l);
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Orders a wave's LDS accesses: its earlier stores are visible to its later
+// loads (one wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Which kernels of a step run: host-driven callers (RUN_ALWAYS) pass their
@@ -78,12 +86,15 @@ __device__ __forceinline__ bool step_runs(const DevState* st, int mode) {
 // bounds used only to size the grid.
 void launch_init_norms(const double* A, int lda, int r0, int m, int n, double* work,
                        double* workold, DevState* st, hipStream_t s);
-// Sorted top-kmax of work[j..n) (value descending, index ascending) and the
-// candidate count (> tau * max), into st.  j, kmax = min(nb, m-j, n-j, KMAX)
-// from st.  tmp_val/tmp_idx hold 2 * half entries, half >=
-// ceil(n/TOPK_CHUNK)*KMAX.  j0 <= j sizes the grid.
+// The pending end of the previous step (see launch_step_end), then the
+// sorted top-kmax of work[j..n) (value descending, index ascending) and the
+// candidate count (> tau * max), into st.  kmax = min(nb, m-j, n-j, KMAX).
+// j0 <= j sizes the grid; tmp_val/tmp_idx hold topk_blocks(n) * KMAX
+// entries; *ticket is 0 between launches.
+int topk_blocks(int n);
 void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, double* tmp_val,
-                 int* tmp_idx, int half, DevState* st, hipStream_t s);
+                 int* tmp_idx, unsigned* ticket, DevState* st, int* ncols_dev, int enable_stop,
+                 double eta, hipStream_t s);
 // Gram of ncols (<= 64) columns over rows [r0, r1): columns are
 // base[(cols ? cols[c] : c) * ld + r].  partial needs ceil((r1-r0)/GRAM_ROWS)
 // * 64 * 64 doubles.  G is 64 x 64 column-major; entries beyond ncols are 0.
@@ -135,7 +146,9 @@ void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const d
 void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* work,
                         double* workold, DevState* st, double tol3z, hipStream_t s);
 // End of a DM step: ncols, the step record and work, the stop test
-// (enable_stop, eta absolute), the next j, done.
+// (enable_stop, eta absolute), the next j, done.  k_norm_update marks it
+// pending; the next step's k_topk runs it, and this launch runs it after the
+// last step queued.
 void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
                      hipStream_t s);
 // Start of the DM loop at column j.
