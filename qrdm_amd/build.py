@@ -45,7 +45,8 @@ def ext_path():
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "panel.hip", "driver.hip"):
+    # kernels.hip and panel.hip compile together through device.hip
+    for src in ("device.hip", "driver.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

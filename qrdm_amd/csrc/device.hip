@@ -1,0 +1,5 @@
+// The device code of libqrdm_amd.so as one translation unit: the kernels of
+// panel.hip combine stages defined in kernels.hip (one launch runs several
+// steps of a DM iteration).
+#include "kernels.hip"
+#include "panel.hip"

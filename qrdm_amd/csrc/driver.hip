@@ -353,22 +353,27 @@ int run_dm(const Args& x) {
         launch_topk(B.work, j0, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.tickets, B.st,
                     B.ncols, enable_stop, eta, s);
         // candidate Gram over rows j..m (cosines need it only when nc > 1;
-        // the kernel reads nc from the state and zero-fills beyond it)
-        launch_gram(A, lda, j0, m, B.st->cand_idx, KMAX, &B.st->nc, B.gram_part, B.G, s, B.st,
-                    RUN_DM);
-        launch_select(B.G, delta, m, n, fast_rows, B.perm, B.st, s);
+        // the kernel reads nc from the state and zero-fills beyond it), then
+        // the greedy acceptance and the permutation plan
+        launch_gram_select(A, lda, j0, m, n, B.gram_part, B.G, delta, fast_rows, B.perm,
+                           B.tickets + 1, B.st, s);
         g_prof.end(ps, s, 0.0);
-        int pp = g_prof.begin(PC_PERM, s);
-        launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
-        g_prof.end(pp, s, 0.0);
         // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) ----------------------
         // Tall panels: Gram-based Householder reconstruction (panel.hip),
-        // which also leaves V and T for the trailing update; short or
-        // degenerate panels: the column-by-column exact panel, then dlarft.
+        // which also leaves V and T for the trailing update, its first
+        // Cholesky beside the column moves; short or degenerate panels: the
+        // column-by-column exact panel, then dlarft.
+        const bool tall = m - j0 >= kFastPanelRows && !force_exact;
+        int pp = g_prof.begin(PC_PERM, s);
+        if (tall)
+          launch_perm_chol1(A, lda, m, B.work, B.jp, B.perm, B.G, tau_dm, B.R1, B.R1inv, B.st, s);
+        else
+          launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
+        g_prof.end(pp, s, 0.0);
         int pq = g_prof.begin(PC_PANEL, s);
-        if (m - j0 >= kFastPanelRows && !force_exact)
-          launch_panel_fast(A, lda, j0, m, tau_dm, x.tau, B.G, B.perm + kRankOff, B.gram_part,
-                            B.R1, B.R1inv, B.G2, B.T, B.M, B.V, B.ldv, B.st, s);
+        if (tall)
+          launch_panel_fast(A, lda, j0, m, tau_dm, x.tau, B.gram_part, B.R1, B.R1inv, B.G2, B.T,
+                            B.M, B.V, B.ldv, B.tickets + 2, B.st, s);
         launch_panel_exact(A, lda, j0, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
         launch_vgram_exact(A, lda, j0, m, B.st, B.V, B.ldv, B.gram_part, s);
         launch_larft(B.gram_part, x.tau, B.st, B.T, s, RUN_EXACT, vgram_exact_slabs(j0, m));

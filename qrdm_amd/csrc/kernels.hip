@@ -407,30 +407,27 @@ __global__ __launch_bounds__(64 * TOPK_WAVES) void k_topk(const double* __restri
     __syncthreads();
     block_merge_lists<TOPK_WAVES>(lv, li, v, ix);
     if (wave == 0) {
-      tmp_val[blockIdx.x * KMAX + lane] = v;
-      tmp_idx[blockIdx.x * KMAX + lane] = ix;
+      st_dev(tmp_val + blockIdx.x * KMAX + lane, v);
+      st_dev(tmp_idx + blockIdx.x * KMAX + lane, ix);
     }
   }
-  __threadfence();
-  if (threadIdx.x == 0) s_last = (atomicAdd(ticket, 1u) == (unsigned)(nblk - 1)) ? 1 : 0;
+  handoff_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = last_to_arrive(ticket, nblk) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
   if (sort) {
     // wave w merges lists w, w + TOPK_WAVES, ... into its own
     v = -1.0;  // a sorted list of padding keys
     ix = 0x7fffffc0 + lane;
     for (int b = wave; b < nblk; b += TOPK_WAVES)
-      wave_merge64(v, ix, tmp_val[b * KMAX + lane], tmp_idx[b * KMAX + lane]);
+      wave_merge64(v, ix, ld_dev(tmp_val + b * KMAX + lane), ld_dev(tmp_idx + b * KMAX + lane));
     lv[wave][lane] = v;
     li[wave][lane] = ix;
     __syncthreads();
     block_merge_lists<TOPK_WAVES>(lv, li, v, ix);
   }
-  if (threadIdx.x == 0) {
-    if (st->pend) step_end_apply(st, ncols_dev, m_all, n_all, enable_stop, eta);
-    *ticket = 0u;
-  }
+  if (threadIdx.x == 0 && st->pend) step_end_apply(st, ncols_dev, m_all, n_all, enable_stop, eta);
   if (sort && wave == 0) {
     // k_max = min(nb, m_sub, n_sub) (src/dgeqrdm_work.c:703), at most KMAX here
     int kmax = nb < KMAX ? nb : KMAX;
@@ -586,12 +583,14 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // 3. the plan: (dst, src) pairs of the moved columns into perm[0..4*KMAX),
 //    the candidate rank of each panel column into perm[4*KMAX..5*KMAX), and
 //    the panel form (Gram-based when fjb >= 2 and m - j >= fast_rows).
-__global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, double delta,
-                                               int m, int n, int fast_rows, int* perm,
-                                               DevState* st) {
+// Run by one wave (threadIdx.x < 64); its LDS steps are ordered by
+// wave_lds_sync, so the rest of the workgroup may be elsewhere.  G is read
+// with device-coherent loads (the launch's other workgroups wrote it).
+__device__ void select_wave(const double* __restrict__ G, double delta, int m, int n,
+                            int fast_rows, int* perm, DevState* st) {
   __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
-  if (st->done) return;
+  __shared__ int s_occ[KMAX], s_slot[KMAX];
   const int t = threadIdx.x;
   const int j = st->j, nc = st->nc, kmax = st->kmax;
   const int nsub = n - j;
@@ -603,11 +602,11 @@ __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, dou
   if (nc > 1) {
     const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
     s_r[t] = rt;
-    __syncthreads();
+    wave_lds_sync();
     unsigned long long conf = 0ull;
 #pragma unroll 8
     for (int q = 0; q < KMAX; q++) {
-      const double c = fabs(G[t + q * KMAX] * s_r[q] * rt);  // G symmetric: G(q, t)
+      const double c = fabs(ld_dev(G + t + q * KMAX) * s_r[q] * rt);  // G symmetric: G(q, t)
       conf |= (q < t && q < nc && c >= delta) ? (1ull << q) : 0ull;
     }
     for (int i = 1; i < nc; i++) {
@@ -630,7 +629,7 @@ __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, dou
     st->sel[spos] = cidx;
     st->selpos[spos] = t;
   }
-  __syncthreads();
+  wave_lds_sync();
   // --- 2. permute_marked's walk, in closed form.  With the accepted
   // positions S (|S| = fjb), jt = n_sub - 1 and the free front slots
   // F = [0, fjb) \ S in increasing order, the walk (host permute_walk)
@@ -644,21 +643,20 @@ __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, dou
   //    going to its position; accepted columns in the front stay.
   // (Checked against the host walk exhaustively for n_sub <= 7 and on random
   // selections up to 300 columns.)  Lane t handles acceptance rank t.
-  __shared__ int s_occ[KMAX], s_slot[KMAX];
   const unsigned long long below = (1ull << t) - 1ull;
   const int jt = nsub - 1;
   const int ps = (t < fjb) ? s_sel[t] : -1;
   s_occ[t] = 0;
-  __syncthreads();
+  wave_lds_sync();
   if (t < fjb && ps < fjb) s_occ[ps] = 1;
-  __syncthreads();
+  wave_lds_sync();
   const unsigned long long freem = __ballot(t < fjb && s_occ[t] == 0);
   if ((freem >> t) & 1ull) s_slot[__popcll(freem & below)] = t;
   const bool zero_sel = s_occ[0] != 0;
   const bool hasA = jt > 0 && __ballot(t < fjb && ps == jt) != 0ull;
   const bool farA = hasA && jt >= fjb;
   const unsigned long long farm = __ballot(t < fjb && ps >= fjb && !(hasA && ps == jt));
-  __syncthreads();
+  wave_lds_sync();
   int mp = ps, um_dst = 0, um_src = 0;
   bool um = false;
   if (t < fjb) {
@@ -705,9 +703,54 @@ __global__ __launch_bounds__(64) void k_select(const double* __restrict__ G, dou
   }
 }
 
-void launch_select(const double* G, double delta, int m, int n, int fast_rows, int* perm,
-                   DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(64), 0, s, G, delta, m, n, fast_rows, perm, st);
+
+// The DM step's candidate Gram reduce (as k_gram_reduce, RUN_DM), then the
+// last workgroup to finish (atomic ticket, reset for the next launch) runs
+// the greedy acceptance and the step's plan on it (select_wave).
+__global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict__ partial,
+                                                       int nslab, double* G, double delta,
+                                                       int m, int n, int fast_rows, int* perm,
+                                                       unsigned* ticket, DevState* st) {
+  __shared__ double part[4][64];
+  __shared__ int s_last;
+  if (st->done) return;
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int pr = e & 63, pc = e >> 6;
+  const int es = (pr > pc) ? pc + pr * KMAX : e;  // symmetric: the upper triangle's value
+  const int q = threadIdx.x >> 6;
+  double sacc = 0.0;
+  for (int b0 = q; b0 < nslab; b0 += 32) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int b = b0 + 4 * u;
+      v[u] = partial[(size_t)(b < nslab ? b : q) * KMAX * KMAX + es];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) sacc += (b0 + 4 * u < nslab) ? v[u] : 0.0;
+  }
+  part[q][threadIdx.x & 63] = sacc;
+  __syncthreads();
+  if (q == 0)
+    st_dev(G + e, (part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                      (part[2][threadIdx.x] + part[3][threadIdx.x]));
+  handoff_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  if (threadIdx.x < 64) select_wave(G, delta, m, n, fast_rows, perm, st);
+}
+
+void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
+                        double* G, double delta, int fast_rows, int* perm, unsigned* ticket,
+                        DevState* st, hipStream_t s) {
+  int nslab = (m - j0 + GRAM_ROWS - 1) / GRAM_ROWS;
+  if (nslab < 1) nslab = 1;
+  hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, A, lda, j0, m, st->cand_idx,
+                     KMAX, &st->nc, partial, st, RUN_DM);
+  hipLaunchKernelGGL(k_reduce_select, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
+                     delta, m, n, fast_rows, perm, ticket, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -719,13 +762,14 @@ void launch_select(const double* G, double delta, int m, int n, int fast_rows, i
 // so no slab races another.
 constexpr int PERM_ROWS = 16;  // rows per workgroup: m / 16 workgroups move the columns
 
-__global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
-                                                    int* jp, const int* __restrict__ perm,
-                                                    const DevState* st) {
+// Row slab blk of the step's column moves (256 threads); slab 0 also moves
+// the work and jpvt entries.
+__device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp,
+                                const int* __restrict__ perm, const DevState* st, int blk) {
   __shared__ double buf[2 * KMAX * PERM_ROWS];
   if (st->done) return;
   const int npair = st->npair;
-  const int rs = blockIdx.x * PERM_ROWS;
+  const int rs = blk * PERM_ROWS;
   // npair <= 2 * KMAX: at most 8 elements per thread (256 threads), all
   // loaded before they are stored
   const int tot = npair * PERM_ROWS;
@@ -746,7 +790,7 @@ __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, d
   }
   double wv = 0.0;
   int jv = 0;
-  const bool side = blockIdx.x == 0 && threadIdx.x < npair;
+  const bool side = blk == 0 && threadIdx.x < npair;
   if (side) {
     wv = work[perm[2 * threadIdx.x + 1]];
     if (jp) jv = jp[perm[2 * threadIdx.x + 1]];
@@ -761,6 +805,14 @@ __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, d
     if (jp) jp[perm[2 * threadIdx.x]] = jv;
   }
 }
+
+__global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
+                                                    int* jp, const int* __restrict__ perm,
+                                                    const DevState* st) {
+  apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x);
+}
+
+int apply_perm_blocks(int m) { return (m + PERM_ROWS - 1) / PERM_ROWS; }
 
 void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
                        DevState* st, hipStream_t s) {

@@ -363,9 +363,8 @@ __device__ __forceinline__ void tri_inv_upper(const double (*Rin)[LDP], bool tra
 // R1 = chol(G) of the panel's Gram (gathered from the candidate Gram through
 // the rank map), stopped at the first pivot that proves the DM truncation
 // fires there; R1^-1.  Identity padding outside fjb.
-__global__ __launch_bounds__(256) void k_chol1(const double* __restrict__ Gc,
-                                               const int* __restrict__ rank, double tau_dm,
-                                               double* R1, double* R1inv, DevState* st) {
+__device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict__ rank,
+                           double tau_dm, double* R1, double* R1inv, DevState* st) {
   __shared__ double S[KMAX][LDP];
   __shared__ double X[KMAX][LDP];
   __shared__ double Y[3][BLK][BLK + 1];
@@ -431,6 +430,27 @@ __global__ __launch_bounds__(256) void k_chol1(const double* __restrict__ Gc,
     R1inv[r + c * KMAX] = (r < k1 && c < k1) ? X[r][c] : 0.0;
   }
   if (tid == 0) st->k1 = k1;
+}
+
+// The step's column moves (workgroups 1..) and, for the Gram-based panel, the
+// first Cholesky (workgroup 0, dispatched first): the Cholesky reads only the
+// candidate Gram, so the two run side by side.
+__global__ __launch_bounds__(256) void k_perm_chol1(double* A, int lda, int m, double* work,
+                                                    int* jp, const int* __restrict__ perm,
+                                                    const double* __restrict__ Gc,
+                                                    double tau_dm, double* R1, double* R1inv,
+                                                    DevState* st) {
+  if (blockIdx.x == 0)
+    chol1_body(Gc, perm + 4 * KMAX, tau_dm, R1, R1inv, st);
+  else
+    apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x - 1);
+}
+
+void launch_perm_chol1(double* A, int lda, int m, double* work, int* jp, const int* perm,
+                       const double* Gc, double tau_dm, double* R1, double* R1inv, DevState* st,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_perm_chol1, dim3(1 + apply_perm_blocks(m)), dim3(256), 0, s, A, lda, m,
+                     work, jp, perm, Gc, tau_dm, R1, R1inv, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -527,11 +547,9 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
 // Reconstruction: see the file header.  Writes the top k x k block of the
 // panel (R above and on the diagonal, V1 below), tau, T and M; or only the
 // fallback flag.  st->fjb_cmp = k.
-__global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
-                                               const double* __restrict__ R1,
-                                               const double* __restrict__ R1inv, double* A,
-                                               int lda, double tau_dm, double* tau,
-                                               double* T_out, double* M_out, DevState* st) {
+__device__ void recon_body(const double* __restrict__ G2, const double* __restrict__ R1,
+                           const double* __restrict__ R1inv, double* A, int lda, double tau_dm,
+                           double* tau, double* T_out, double* M_out, DevState* st) {
   __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
   __shared__ double SA[KMAX][LDP];  // R^-1
   __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
@@ -551,7 +569,7 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
       // e walks rows fastest: coalesced column-major reads
       const int e = tid + 256 * q, r = e & 63, c = e >> 6;
       const bool in = r < k1 && c < k1;
-      gx[q] = in ? G2[r + c * KMAX] : (r == c ? 1.0 : 0.0);
+      gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);  // from the launch's reduce
       gr[q] = R1[r + c * KMAX];
     }
 #pragma unroll
@@ -741,6 +759,45 @@ __global__ __launch_bounds__(256) void k_recon(const double* __restrict__ G2,
   }
 }
 
+// G2 = the sum of pass 2's partial Grams (as k_gram_reduce), then the last
+// workgroup to finish (atomic ticket, reset for the next launch) runs the
+// reconstruction on it.
+__global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__ partial,
+                                                      int nslab, double* G2,
+                                                      const double* __restrict__ R1,
+                                                      const double* __restrict__ R1inv,
+                                                      double* A, int lda, double tau_dm,
+                                                      double* tau, double* T_out, double* M_out,
+                                                      unsigned* ticket, DevState* st) {
+  __shared__ double part[4][64];
+  __shared__ int s_last;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  double sacc = 0.0;
+  for (int b0 = q; b0 < nslab; b0 += 32) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int b = b0 + 4 * u;
+      v[u] = partial[(size_t)(b < nslab ? b : q) * KMAX * KMAX + e];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) sacc += (b0 + 4 * u < nslab) ? v[u] : 0.0;
+  }
+  part[q][threadIdx.x & 63] = sacc;
+  __syncthreads();
+  if (q == 0)
+    st_dev(G2 + e, (part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                       (part[2][threadIdx.x] + part[3][threadIdx.x]));
+  handoff_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  recon_body(G2, R1, R1inv, A, lda, tau_dm, tau, T_out, M_out, st);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 3: V2 = P_bottom M on a 64-row slab; writes V2 into the panel below the
 // top block and the dense zero-padded V (rows [j, m), KMAX columns) with V1's
@@ -801,15 +858,13 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
 }
 
 void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
-                       const double* Gc, const int* rank, double* gram_part, double* R1,
-                       double* R1inv, double* G2, double* T, double* M, double* V, int ldv,
-                       DevState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_chol1, dim3(1), dim3(256), 0, s, Gc, rank, tau_dm, R1, R1inv, st);
+                       double* gram_part, double* R1, double* R1inv, double* G2, double* T,
+                       double* M, double* V, int ldv, unsigned* ticket, DevState* st,
+                       hipStream_t s) {
   const int nslab = (m - j0 + PR_ROWS - 1) / PR_ROWS;
   hipLaunchKernelGGL(k_qgram, dim3(nslab), dim3(256), 0, s, A, lda, m, R1inv, st, gram_part);
-  launch_gram_reduce(gram_part, nslab, G2, s, st, RUN_FAST);
-  hipLaunchKernelGGL(k_recon, dim3(1), dim3(256), 0, s, G2, R1, R1inv, A, lda, tau_dm, tau, T, M,
-                     st);
+  hipLaunchKernelGGL(k_reduce_recon, dim3(KMAX * KMAX / 64), dim3(256), 0, s, gram_part, nslab,
+                     G2, R1, R1inv, A, lda, tau_dm, tau, T, M, ticket, st);
   hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, m, M, st, V, ldv);
 }
 

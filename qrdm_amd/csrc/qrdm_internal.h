@@ -66,6 +66,34 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Data handed from the workgroups of a launch to its last workgroup (atomic
+// ticket): stores and loads coherent at device scope (performed at the
+// device's coherence point, past every XCD's L2), so the ticket needs no
+// cache-wide write-back; each wave waits for its own stores to complete
+// (handoff_wait) before the barrier that precedes the ticket.
+__device__ __forceinline__ void st_dev(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_dev(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_dev(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void handoff_wait() { __builtin_amdgcn_s_waitcnt(0); }
+// Called by one thread after every wave's handoff_wait and a barrier: whether
+// this workgroup is the last of n to arrive; the last one resets the ticket
+// for the next launch.
+__device__ __forceinline__ bool last_to_arrive(unsigned* ticket, unsigned n) {
+  const bool last =
+      __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == n - 1;
+  if (last) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return last;
+}
+
 // Which kernels of a step run: host-driven callers (RUN_ALWAYS) pass their
 // own geometry; the DM loop's kernels (RUN_DM) take j from DevState and
 // return once the loop is done; RUN_EXACT / RUN_FAST kernels belong to one
@@ -104,11 +132,14 @@ void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, d
 void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, int ncols_max,
                  const int* ncols_dev, double* partial, double* G, hipStream_t s,
                  const DevState* st = nullptr, int mode = RUN_ALWAYS);
-// Greedy cosine acceptance, then the step's plan: permute_marked's walk, the
-// (dst, src) column pairs into perm[0..4*KMAX), the panel columns' candidate
-// ranks into perm[4*KMAX..5*KMAX), the panel form.
-void launch_select(const double* G, double delta, int m, int n, int fast_rows, int* perm,
-                   DevState* st, hipStream_t s);
+// The step's candidate Gram over rows j..m (RUN_DM geometry; j0 <= j sizes
+// the grid), then, in the last workgroup of the reduce (*ticket is 0 between
+// launches), the greedy cosine acceptance and the step's plan: permute_marked's
+// walk, the (dst, src) column pairs into perm[0..4*KMAX), the panel columns'
+// candidate ranks into perm[4*KMAX..5*KMAX), the panel form.
+void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
+                        double* G, double delta, int fast_rows, int* perm, unsigned* ticket,
+                        DevState* st, hipStream_t s);
 // Moves the step's columns (st->npair pairs in perm), their work entries and
 // jpvt entries (jp may be null).
 void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
@@ -158,13 +189,19 @@ void launch_set_kk(DevState* st, int kk, hipStream_t s);
 void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s,
                         const DevState* st = nullptr, int mode = RUN_ALWAYS);
 // Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST
-// kernels: j, fjb from st.  Gc is the step's candidate Gram (64 x 64,
-// candidate order) and rank[p] the candidate rank of panel column p.  j0 <=
-// j sizes the grids.  Sets st->fjb_cmp and st->c0, or the fallback flag.
+// kernels: j, fjb from st.  launch_perm_chol1 moves the step's columns (as
+// launch_apply_perm) and factors the panel's Gram, gathered from the candidate
+// Gram Gc (64 x 64, candidate order) by the panel columns' candidate ranks
+// perm[4*KMAX..]; launch_panel_fast then runs the rest of the panel.  j0 <= j
+// sizes the grids.  Sets st->fjb_cmp and st->c0, or the fallback flag.
+void launch_perm_chol1(double* A, int lda, int m, double* work, int* jp, const int* perm,
+                       const double* Gc, double tau_dm, double* R1, double* R1inv, DevState* st,
+                       hipStream_t s);
 void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
-                       const double* Gc, const int* rank, double* gram_part, double* R1,
-                       double* R1inv, double* G2, double* T, double* M, double* V, int ldv,
-                       DevState* st, hipStream_t s);
+                       double* gram_part, double* R1, double* R1inv, double* G2, double* T,
+                       double* M, double* V, int ldv, unsigned* ticket, DevState* st,
+                       hipStream_t s);
+int apply_perm_blocks(int m);
 void launch_clear_flag(DevState* st, int bits, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
 

@@ -217,6 +217,17 @@ def test_sweep_vs_oracle(gpu, oracle, seed):
     assert_parity(got, ref)
 
 
+@pytest.mark.parametrize("kind,m,n", [("gauss", 700, 2200), ("lowrank", 1500, 1500),
+                                      ("dupcols", 600, 3000), ("int", 2100, 1100)])
+def test_wide_candidate_lists(gpu, oracle, kind, m, n):
+    """More than 512 trailing columns: the candidate top-k merges the lists
+    of several workgroups (and the step's end runs inside it)."""
+    a = inputs.small_case(kind, m, n, 31 * m + n)
+    say("wide", kind, m, n)
+    got = run_host(a)
+    assert_parity(got, oracle.dgeqrdm(a))
+
+
 @pytest.mark.parametrize("shape", [(1, 1), (1, 9), (9, 1), (2, 2), (64, 64), (65, 64),
                                    (64, 65), (257, 129), (130, 600)])
 def test_edge_shapes(gpu, oracle, shape):
