@@ -113,13 +113,15 @@ int qrdm_permute_walk(int ncols, const int *sel, int nsel, int *swaps);
  * HIP events (on the stream it launches on) around each class of kernel and
  * accumulates, per class, the summed event time in ms, the launch count and
  * the algorithmic work of those launches.  Classes:
- *   0 trailing update, first half (W = T^T V^T C)      work = flops
+ *   0 trailing update, partial products V^T C (k_gemm1) work = flops
  *   1 norm downdate/recompute                          work = HBM bytes
  *   2 DM selection (top-k, candidate Gram, greedy)     work = HBM bytes
  *   3 panel (Householder + V + T)                      work = flops
- *   4 column permutation                               work = HBM bytes
+ *   4 column permutation (on tall panels with the      work = HBM bytes
+ *     panel's first Cholesky beside it)
  *   5 initial norms                                    work = HBM bytes
  *   6 trailing update, second half (C -= V W)          work = HBM bytes
+ *   7 trailing update, W = T^T (sum of the partials)   work = flops of T^T
  * qrdm_profile_read fills out[3*c + {0,1,2}] = {ms, launches, work} for
  * c < nclass and returns the number of classes.  No counterpart in the
  * reference (it has no timing facility).

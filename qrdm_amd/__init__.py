@@ -33,7 +33,7 @@ EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrf
            "qrdm_last_error", "qrdm_version")
 
 PROFILE_CLASSES = ("larfb_w", "norm_update", "select", "panel", "permute", "init_norms",
-                   "larfb_apply")
+                   "larfb_apply", "larfb_sum")
 
 
 def lib():

@@ -50,8 +50,9 @@ struct HipError {
 struct Workspace {
   size_t bytes = 0;
   char* base = nullptr;
-  void* pinned = nullptr;  // host staging for the per-step readback
+  void* pinned = nullptr;  // host staging for the per-batch state readback (two slots)
   size_t pinned_bytes = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // completion of each slot's readback
 };
 
 std::mutex g_mu;
@@ -59,9 +60,9 @@ Workspace g_ws[64];
 
 // Kernel-time accounting (qrdm_profile_*): HIP events around each class of
 // kernels on the launch stream, resolved after the call's final sync.
-constexpr int kNumClasses = 7;
+constexpr int kNumClasses = 8;
 enum { PC_LARFB = 0, PC_NORM = 1, PC_SELECT = 2, PC_PANEL = 3, PC_PERM = 4, PC_INIT = 5,
-       PC_APPLY = 6 };
+       PC_APPLY = 6, PC_WSUM = 7 };
 struct Profiler {
   bool on = false;
   unsigned mask = 0xffffffffu;  // bit c: class c is timed
@@ -253,8 +254,9 @@ Workspace& workspace_for(int m, int n, Buffers* b) {
     ws.bytes = need;
   }
   if (!ws.pinned) {
-    ws.pinned_bytes = sizeof(DevState) + 4096;
+    ws.pinned_bytes = 2 * sizeof(DevState) + 4096;
     HIPCHK(hipHostMalloc(&ws.pinned, ws.pinned_bytes, hipHostMallocDefault));
+    for (int q = 0; q < 2; q++) HIPCHK(hipEventCreateWithFlags(&ws.ev[q], hipEventDisableTiming));
   }
   layout_buffers(m, n, ws.base, b);
   // The buffer offsets depend on (m, n): clear the trailing update's ticket
@@ -346,57 +348,112 @@ int run_dm(const Args& x) {
     launch_dm_begin(B.st, nfxd, minmn, s);
     const int batch = x.trace ? 1 : kBatch;
     int j0 = nfxd, it = 0;
-    while (j0 < minmn) {
-      for (int b = 0; b < batch; b++) {
-        // --- DM_perm (src/dgeqrdm_work.c:280-418) -------------------------
-        int ps = g_prof.begin(PC_SELECT, s);
-        launch_topk(B.work, j0, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.tickets, B.st,
-                    B.ncols, enable_stop, eta, s);
-        // candidate Gram over rows j..m (cosines need it only when nc > 1;
-        // the kernel reads nc from the state and zero-fills beyond it), then
-        // the greedy acceptance and the permutation plan
-        launch_gram_select(A, lda, j0, m, n, B.gram_part, B.G, delta, fast_rows, B.perm,
-                           B.tickets + 1, B.st, s);
-        g_prof.end(ps, s, 0.0);
-        // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) ----------------------
-        // Tall panels: Gram-based Householder reconstruction (panel.hip),
-        // which also leaves V and T for the trailing update, its first
-        // Cholesky beside the column moves; short or degenerate panels: the
-        // column-by-column exact panel, then dlarft.
-        const bool tall = m - j0 >= kFastPanelRows && !force_exact;
-        int pp = g_prof.begin(PC_PERM, s);
-        if (tall)
-          launch_perm_chol1(A, lda, m, B.work, B.jp, B.perm, B.G, tau_dm, B.R1, B.R1inv, B.st, s);
-        else
-          launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
-        g_prof.end(pp, s, 0.0);
+    // the exact panel and dlarft of the step at column jl (RUN_EXACT: they
+    // run when the step's panel is not the Gram-based one, or that fell back)
+    auto queue_exact = [&](int jl) {
+      int pq = g_prof.begin(PC_PANEL, s);
+      launch_panel_exact(A, lda, jl, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
+      launch_vgram_exact(A, lda, jl, m, B.st, B.V, B.ldv, B.gram_part, s);
+      launch_larft(B.gram_part, x.tau, B.st, B.T, s, RUN_EXACT, vgram_exact_slabs(jl, m));
+      g_prof.end(pq, s, 0.0);
+    };
+    // --- dlarfb (src/dgeqrdm_work.c:762-767) from c0 (set by the panel),
+    // norm_update (src/dgeqrdm_work.c:774-778); the stop test and the next j
+    // follow in the step's end
+    auto queue_update = [&](int jl) {
+      int pl = g_prof.begin(PC_LARFB, s);
+      launch_gemm1(A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
+      g_prof.end(pl, s, 0.0);
+      int pw = g_prof.begin(PC_WSUM, s);
+      launch_wsum(jl, m, jl + 1, n, B.T, 1, B.Wp, B.W, B.nsplit, s, B.st);
+      g_prof.end(pw, s, 0.0);
+      int pa = g_prof.begin(PC_APPLY, s);
+      launch_larfb_apply(A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.W, s, B.st);
+      g_prof.end(pa, s, 0.0);
+      int pn = g_prof.begin(PC_NORM, s);
+      launch_norm_update(A, lda, jl, m, n, B.work, B.workold, B.st, tol3z, s);
+      g_prof.end(pn, s, 0.0);
+    };
+    // one DM step from column jl (a lower bound of the step's j)
+    auto queue_step = [&](int jl, bool tall) {
+      // --- DM_perm (src/dgeqrdm_work.c:280-418) ---------------------------
+      int ps = g_prof.begin(PC_SELECT, s);
+      launch_topk(B.work, jl, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.tickets, B.st, B.ncols,
+                  enable_stop, eta, s);
+      // candidate Gram over rows j..m (cosines need it only when nc > 1;
+      // the kernel reads nc from the state and zero-fills beyond it), then
+      // the greedy acceptance and the permutation plan
+      launch_gram_select(A, lda, jl, m, n, B.gram_part, B.G, delta, fast_rows, tall ? 0 : 1,
+                         B.perm, B.tickets + 1, B.st, s);
+      g_prof.end(ps, s, 0.0);
+      // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) ------------------------
+      int pp = g_prof.begin(PC_PERM, s);
+      if (tall)
+        launch_perm_chol1(A, lda, m, B.work, B.jp, B.perm, B.G, tau_dm, B.R1, B.R1inv, B.st, s);
+      else
+        launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
+      g_prof.end(pp, s, 0.0);
+      if (tall) {
         int pq = g_prof.begin(PC_PANEL, s);
-        if (tall)
-          launch_panel_fast(A, lda, j0, m, tau_dm, x.tau, B.gram_part, B.R1, B.R1inv, B.G2, B.T,
-                            B.M, B.V, B.ldv, B.tickets + 2, B.st, s);
-        launch_panel_exact(A, lda, j0, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
-        launch_vgram_exact(A, lda, j0, m, B.st, B.V, B.ldv, B.gram_part, s);
-        launch_larft(B.gram_part, x.tau, B.st, B.T, s, RUN_EXACT, vgram_exact_slabs(j0, m));
+        launch_panel_fast(A, lda, jl, m, tau_dm, x.tau, B.gram_part, B.R1, B.R1inv, B.G2, B.T,
+                          B.M, B.V, B.ldv, B.tickets + 2, B.st, s);
         g_prof.end(pq, s, 0.0);
-        // --- dlarfb (src/dgeqrdm_work.c:762-767) from c0 (set by the panel)
-        int pl = g_prof.begin(PC_LARFB, s);
-        launch_larfb_w(A, lda, j0, m, j0 + 1, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit,
-                       B.tickets, s, B.st);
-        g_prof.end(pl, s, 0.0);
-        int pa = g_prof.begin(PC_APPLY, s);
-        launch_larfb_apply(A, lda, j0, m, j0 + 1, n, B.V, B.ldv, B.W, s, B.st);
-        g_prof.end(pa, s, 0.0);
-        // --- norm_update (src/dgeqrdm_work.c:774-778), stop test, next j ---
-        int pn = g_prof.begin(PC_NORM, s);
-        launch_norm_update(A, lda, j0, m, n, B.work, B.workold, B.st, tol3z, s);
-        g_prof.end(pn, s, 0.0);
+      } else {
+        queue_exact(jl);
       }
+      queue_update(jl);
+    };
+    // a batch of steps from column jl, the pending step end, and the state
+    // read back into slot q
+    DevState* hb[2] = {hst, hst + 1};
+    auto readback = [&](int q) {
       launch_step_end(B.st, B.ncols, m, n, enable_stop, eta, s);
-      HIPCHK(hipMemcpyAsync(hst, B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      if (x.trace && hst->it > it && hst->it - 1 < x.ntrace) {
-        const StepRec& l = hst->last;
-        qrdm_step& r = x.trace[hst->it - 1];
+      HIPCHK(hipMemcpyAsync(hb[q], B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipEventRecord(ws.ev[q], s));
+    };
+    auto queue_batch = [&](int jl, int q) {
+      // Tall panels: Gram-based Householder reconstruction (panel.hip), which
+      // also leaves V and T for the trailing update, its first Cholesky beside
+      // the column moves; the exact panel is not queued, and a step that
+      // needs it (short panel, fjb < 2, or the Gram-based form falling back)
+      // halts until the host resumes it.  Short panels: the column-by-column
+      // exact panel, then dlarft.
+      const bool tall = m - jl >= kFastPanelRows && !force_exact;
+      for (int b = 0; b < batch; b++) queue_step(jl, tall);
+      readback(q);
+    };
+    auto wait_slot = [&](int q) {
+      HIPCHK(hipEventSynchronize(ws.ev[q]));
+      return hb[q];
+    };
+    // While the batch being waited for cannot end the loop (its first column
+    // plus a batch of KMAX-wide steps stays below min(m, n)), the next batch
+    // is queued before its state is read back, so the device does not idle
+    // through the readback; its grids are sized from the same lower bound.
+    int q = 0;
+    queue_batch(j0, q);
+    while (true) {
+      const bool ahead = !x.trace && j0 + batch * KMAX < minmn;
+      if (ahead) queue_batch(j0, q ^ 1);
+      DevState* h = wait_slot(q);
+      bool in_flight = ahead;
+      if (h->halt) {
+        // the halted step (at h->j, selection and column moves done)
+        // continues with the exact panel, after the batch queued behind it
+        // (which the halt stops at once)
+        if (in_flight) wait_slot(q ^ 1);
+        in_flight = false;
+        const int jh = h->j;
+        launch_resume(B.st, s);
+        queue_exact(jh);
+        queue_update(jh);
+        readback(q);
+        h = wait_slot(q);
+        if (h->halt) throw HipError{hipErrorUnknown, "resumed DM step halted again"};
+      }
+      if (x.trace && h->it > it && h->it - 1 < x.ntrace) {
+        const StepRec& l = h->last;
+        qrdm_step& r = x.trace[h->it - 1];
         memset(&r, 0, sizeof(r));
         r.j = l.j;
         r.kmax = l.kmax;
@@ -406,11 +463,15 @@ int run_dm(const Args& x) {
         r.nswap = l.nswap;
         r.panel = l.panel;
         r.maxnrm = l.maxnrm;
-        for (int q = 0; q < l.fjb && q < 64; q++) r.sel[q] = hst->sel[q];
+        for (int t = 0; t < l.fjb && t < 64; t++) r.sel[t] = h->sel[t];
       }
-      it = hst->it;
-      if (hst->done) break;
-      j0 = hst->j;
+      it = h->it;
+      if (h->done) break;  // a batch queued ahead returns at once; the final sync drains it
+      j0 = h->j;
+      if (in_flight)
+        q ^= 1;  // the batch queued ahead is the one to wait for next
+      else
+        queue_batch(j0, q);
     }
     if (x.nsteps) *x.nsteps = it;
     HIPCHK(hipMemcpyAsync(x.jpvt, B.jp, sizeof(int) * n, hipMemcpyDeviceToHost, s));
@@ -426,6 +487,7 @@ int run_dm(const Args& x) {
     g_prof.work[PC_PANEL] += w.panel;
     g_prof.work[PC_LARFB] += w.gemm1;
     g_prof.work[PC_APPLY] += w.gemm2;
+    g_prof.work[PC_WSUM] += w.wsum;
     g_prof.work[PC_NORM] += w.norm + 8.0 * (double)hst->recomp_rows;
   }
   return info;

@@ -345,7 +345,8 @@ __device__ void step_end_apply(DevState* st, int* ncols_dev, int m, int n, int e
   w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
   w.perm += 16.0 * (double)m * st->npair;
   w.panel += 2.0 * msub * (double)fjb * fjb;
-  w.gemm1 += 2.0 * msub * nct * kk + 2.0 * kk * (double)kk * nct;
+  w.gemm1 += 2.0 * msub * nct * kk;
+  w.wsum += 2.0 * kk * (double)kk * nct;
   w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
   w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
   const int done = step_done(st, m, n, enable_stop, eta);  // from j before the advance
@@ -384,7 +385,9 @@ __global__ __launch_bounds__(64 * TOPK_WAVES) void k_topk(const double* __restri
       j += st->fjb_cmp;
     }
     s_j = j;
-    s_done = st->done;  // loop already over before this launch: nothing to do
+    s_done = dm_stopped(st);  // loop over or halted before this launch: nothing to do
+    // a halted step's successors stop before their column moves as well
+    if (blockIdx.x == 0 && st->halt == 2) st->halt = 1;
     s_end = done;       // the loop ends with the pending step: no sort
   }
   __syncthreads();
@@ -587,7 +590,7 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // wave_lds_sync, so the rest of the workgroup may be elsewhere.  G is read
 // with device-coherent loads (the launch's other workgroups wrote it).
 __device__ void select_wave(const double* __restrict__ G, double delta, int m, int n,
-                            int fast_rows, int* perm, DevState* st) {
+                            int fast_rows, int exact_queued, int* perm, DevState* st) {
   __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
   __shared__ int s_occ[KMAX], s_slot[KMAX];
@@ -699,7 +702,9 @@ __device__ void select_wave(const double* __restrict__ G, double delta, int m, i
     st->fjb = fjb;
     st->npair = __popcll(bm) + __popcll(bu);
     st->nswap = nswap;
-    st->fast = (fjb >= 2 && m - j >= fast_rows) ? 1 : 0;
+    const int fast = (fjb >= 2 && m - j >= fast_rows) ? 1 : 0;
+    st->fast = fast;
+    if (!fast && !exact_queued) st->halt = 2;
   }
 }
 
@@ -709,11 +714,12 @@ __device__ void select_wave(const double* __restrict__ G, double delta, int m, i
 // the greedy acceptance and the step's plan on it (select_wave).
 __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict__ partial,
                                                        int nslab, double* G, double delta,
-                                                       int m, int n, int fast_rows, int* perm,
+                                                       int m, int n, int fast_rows,
+                                                       int exact_queued, int* perm,
                                                        unsigned* ticket, DevState* st) {
   __shared__ double part[4][64];
   __shared__ int s_last;
-  if (st->done) return;
+  if (dm_stopped(st)) return;
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
   const int pr = e & 63, pc = e >> 6;
   const int es = (pr > pc) ? pc + pr * KMAX : e;  // symmetric: the upper triangle's value
@@ -739,18 +745,18 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
   if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  if (threadIdx.x < 64) select_wave(G, delta, m, n, fast_rows, perm, st);
+  if (threadIdx.x < 64) select_wave(G, delta, m, n, fast_rows, exact_queued, perm, st);
 }
 
 void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
-                        double* G, double delta, int fast_rows, int* perm, unsigned* ticket,
-                        DevState* st, hipStream_t s) {
+                        double* G, double delta, int fast_rows, int exact_queued, int* perm,
+                        unsigned* ticket, DevState* st, hipStream_t s) {
   int nslab = (m - j0 + GRAM_ROWS - 1) / GRAM_ROWS;
   if (nslab < 1) nslab = 1;
   hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, A, lda, j0, m, st->cand_idx,
                      KMAX, &st->nc, partial, st, RUN_DM);
   hipLaunchKernelGGL(k_reduce_select, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
-                     delta, m, n, fast_rows, perm, ticket, st);
+                     delta, m, n, fast_rows, exact_queued, perm, ticket, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -767,7 +773,7 @@ constexpr int PERM_ROWS = 16;  // rows per workgroup: m / 16 workgroups move the
 __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp,
                                 const int* __restrict__ perm, const DevState* st, int blk) {
   __shared__ double buf[2 * KMAX * PERM_ROWS];
-  if (st->done) return;
+  if (st->done || st->halt == 1) return;  // halt 2: this step's moves still run
   const int npair = st->npair;
   const int rs = blk * PERM_ROWS;
   // npair <= 2 * KMAX: at most 8 elements per thread (256 threads), all
@@ -1083,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
                                                const DevState* st) {
   __shared__ __align__(32) double Vs[2][KMAX][16 + G1_VPAD];
   if (st) {
-    if (st->done) return;
+    if (dm_stopped(st)) return;
     j = st->j;
     c0 = st->c0;
     ncol = ldw - c0;
@@ -1211,7 +1217,7 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wp, int
   __shared__ double Ts[KMAX][KMAX + 1];
   __shared__ double Ws[KMAX][KMAX + 1];
   if (st) {
-    if (st->done) return;
+    if (dm_stopped(st)) return;
     c0 = st->c0;
     ncol = ldw - c0;
   }
@@ -1277,7 +1283,7 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
                                                const DevState* st) {
   __shared__ double Vs[64][KMAX + 1];
   if (st) {  // the DM loop: geometry of the current step, grid sized for the lowest j
-    if (st->done) return;
+    if (dm_stopped(st)) return;
     j = st->j;
     c0 = st->c0;
     ncol = ldw - c0;
@@ -1342,17 +1348,17 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
   }
 }
 
-void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                    const double* T, int trans, double* Wp, double* W, int nsplit,
-                    unsigned* tickets, hipStream_t s, const DevState* st) {
-  (void)tickets;
-  // with st: j, c0 are lower bounds (c0 >= j + 1 then) of the step's values
-  const int ncol = n - c0, mrow = m - j;
-  if (ncol <= 0 || mrow <= 0) return;
-  const int ldw = n;
-  const int cgroups = (ncol + 255) / 256;
-  // about one workgroup per CU: split the rows when there are few column groups
-  int want = (256 + cgroups - 1) / cgroups;
+// Row slabs of k_gemm1's split reduction for columns [c0, n) and rows
+// [j & ~(G1_CHUNK - 1), m): about one workgroup per CU.
+struct G1Geom {
+  int cgroups, ns, slab;
+};
+
+static G1Geom gemm1_geom(int j, int m, int c0, int n, int nsplit) {
+  const int ncol = n - c0;
+  G1Geom g;
+  g.cgroups = (ncol + 255) / 256;
+  int want = (256 + g.cgroups - 1) / g.cgroups;
   if (want > nsplit) want = nsplit;
   if (want > G1_MAX_SPLIT) want = G1_MAX_SPLIT;
   const int base = j & ~(G1_CHUNK - 1);
@@ -1360,19 +1366,44 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
   int slab = (span + want - 1) / want;
   slab = ((slab + G1_CHUNK - 1) / G1_CHUNK) * G1_CHUNK;
   if (slab < G1_CHUNK) slab = G1_CHUNK;
-  const int ns = (span + slab - 1) / slab;
+  g.slab = slab;
+  g.ns = (span + slab - 1) / slab;
+  return g;
+}
+
+void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                  double* Wp, int nsplit, hipStream_t s, const DevState* st) {
+  // with st: j, c0 are lower bounds (c0 >= j + 1 then) of the step's values
+  const int ncol = n - c0, mrow = m - j;
+  if (ncol <= 0 || mrow <= 0) return;
+  const G1Geom g = gemm1_geom(j, m, c0, n, nsplit);
   // 32-byte vector loads need 32-byte aligned columns of at least 4 rows
   const int aligned = (lda % 4 == 0) && (ldv % 4 == 0) && m >= 4 &&
                       ((reinterpret_cast<uintptr_t>(A) & 31) == 0) &&
                       ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
   if (aligned)
-    hipLaunchKernelGGL(k_gemm1<true>, dim3(cgroups, ns), dim3(256), 0, s, A, lda, j, m, c0, ncol,
-                       V, ldv, slab, Wp, ldw, st);
+    hipLaunchKernelGGL(k_gemm1<true>, dim3(g.cgroups, g.ns), dim3(256), 0, s, A, lda, j, m, c0,
+                       ncol, V, ldv, g.slab, Wp, n, st);
   else
-    hipLaunchKernelGGL(k_gemm1<false>, dim3(cgroups, ns), dim3(256), 0, s, A, lda, j, m, c0,
-                       ncol, V, ldv, slab, Wp, ldw, st);
-  hipLaunchKernelGGL(k_wsum, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, ns, ldw, c0, ncol, T,
+    hipLaunchKernelGGL(k_gemm1<false>, dim3(g.cgroups, g.ns), dim3(256), 0, s, A, lda, j, m, c0,
+                       ncol, V, ldv, g.slab, Wp, n, st);
+}
+
+void launch_wsum(int j, int m, int c0, int n, const double* T, int trans, const double* Wp,
+                 double* W, int nsplit, hipStream_t s, const DevState* st) {
+  const int ncol = n - c0, mrow = m - j;
+  if (ncol <= 0 || mrow <= 0) return;
+  const G1Geom g = gemm1_geom(j, m, c0, n, nsplit);
+  hipLaunchKernelGGL(k_wsum, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, g.ns, n, c0, ncol, T,
                      trans, W, st);
+}
+
+void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                    const double* T, int trans, double* Wp, double* W, int nsplit,
+                    unsigned* tickets, hipStream_t s, const DevState* st) {
+  (void)tickets;
+  launch_gemm1(A, lda, j, m, c0, n, V, ldv, Wp, nsplit, s, st);
+  launch_wsum(j, m, c0, n, T, trans, Wp, W, nsplit, s, st);
 }
 
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
@@ -1402,7 +1433,7 @@ __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ 
                                                      double* workold, DevState* st,
                                                      double tol3z) {
   __shared__ unsigned long long s_rows[4];
-  if (st->done) return;
+  if (dm_stopped(st)) return;
   // the step's end is pending from here on (run by the next k_topk or k_step_end)
   if (blockIdx.x == 0 && threadIdx.x == 0) st->pend = 1;
   const int j = st->j;
@@ -1470,12 +1501,13 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
 
 __global__ void k_dm_begin(DevState* st, int j, int minmn) {
   st->pend = 0;
+  st->halt = 0;
   st->j = j;
   st->it = 0;
   st->done = (j >= minmn) ? 1 : 0;
   st->maxnrm_bits = 0ull;
   st->flags &= ~2;
-  st->work = StepWork{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  st->work = StepWork{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 }
 
 void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s) {
@@ -1491,6 +1523,12 @@ void launch_set_kk(DevState* st, int kk, hipStream_t s) {
 }
 
 __global__ void k_clear_flag(DevState* st, int bits) { st->flags &= ~bits; }
+
+__global__ void k_resume(DevState* st) { st->halt = 0; }
+
+void launch_resume(DevState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_resume, dim3(1), dim3(1), 0, s, st);
+}
 
 void launch_clear_flag(DevState* st, int bits, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, st, bits);

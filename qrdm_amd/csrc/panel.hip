@@ -412,7 +412,10 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
       },
       rinv, &s_done, &s_why, &why);
   if (why == 2) {
-    if (tid == 0) st->flags |= 2;
+    if (tid == 0) {
+      st->flags |= 2;  // the exact panel takes the step over
+      st->halt = 2;
+    }
     return;
   }
   // R1 (zero outside k1) out; identity-padded R1 in S for the inverse
@@ -673,7 +676,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
         SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv,
         &s_done, &s_why, &why);
     if (why) {
-      if (tid == 0) st->flags |= 2;
+      if (tid == 0) {
+        st->flags |= 2;  // the exact panel takes the step over
+        st->halt = 2;
+      }
       return;
     }
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
@@ -700,7 +706,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SR[0][0] : 5e-14;
   const int k = lu_signed_blocked(SW, k1, SR, thres, dsg, tl, scv, &s_done, &s_why, &why);
   if (why == 2) {
-    if (tid == 0) st->flags |= 2;
+    if (tid == 0) {
+      st->flags |= 2;  // the exact panel takes the step over
+      st->halt = 2;
+    }
     return;
   }
   // 6. top block (R rows scaled by the signs, V1 below), tau; U -> SX

@@ -18,7 +18,7 @@ struct StepRec {
 // Algorithmic work of the steps run so far, per kernel class (flops or HBM
 // bytes), accumulated on the device for the kernel-time accounting.
 struct StepWork {
-  double select, perm, panel, gemm1, gemm2, norm;
+  double select, perm, panel, gemm1, gemm2, norm, wsum;
 };
 
 // Per-factorisation state shared by the kernels.  Lives in device memory.
@@ -46,6 +46,10 @@ struct DevState {
   int npair;   // (dst, src) column pairs of the current step's permutation
   int nswap;   // swaps made by the current step's permutation walk
   int pend;    // 1: the current step's end (k_step_end's work) has not run yet
+  int halt;    // the step needs the exact panel, which was not queued; the DM
+               // kernels return until the host resumes the step.  2: the step's
+               // column moves still run (set before them); the next step's
+               // top-k turns it into 1, which stops the moves too
   StepRec last;
   StepWork work;
 };
@@ -100,9 +104,12 @@ __device__ __forceinline__ bool last_to_arrive(unsigned* ticket, unsigned n) {
 // of the two panel forms and return unless the step uses that form.
 enum RunMode { RUN_ALWAYS = 0, RUN_DM = 1, RUN_EXACT = 2, RUN_FAST = 3 };
 
+// The DM loop has ended, or waits for the host to resume a halted step.
+__device__ __forceinline__ bool dm_stopped(const DevState* st) { return st->done || st->halt; }
+
 __device__ __forceinline__ bool step_runs(const DevState* st, int mode) {
   if (mode == RUN_ALWAYS) return true;
-  if (st->done) return false;
+  if (dm_stopped(st)) return false;
   if (mode == RUN_EXACT) return !st->fast || (st->flags & 2);
   if (mode == RUN_FAST) return st->fast && !(st->flags & 2);
   return true;
@@ -137,9 +144,11 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // launches), the greedy cosine acceptance and the step's plan: permute_marked's
 // walk, the (dst, src) column pairs into perm[0..4*KMAX), the panel columns'
 // candidate ranks into perm[4*KMAX..5*KMAX), the panel form.
+// exact_queued = 0: the step's exact-panel kernels are not queued, and a step
+// that needs them (short panel, fjb < 2) halts for the host to resume it.
 void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
-                        double* G, double delta, int fast_rows, int* perm, unsigned* ticket,
-                        DevState* st, hipStream_t s);
+                        double* G, double delta, int fast_rows, int exact_queued, int* perm,
+                        unsigned* ticket, DevState* st, hipStream_t s);
 // Moves the step's columns (st->npair pairs in perm), their work entries and
 // jpvt entries (jp may be null).
 void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
@@ -172,6 +181,12 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
                     unsigned* tickets, hipStream_t s, const DevState* st = nullptr);
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                         const double* W, hipStream_t s, const DevState* st = nullptr);
+// launch_larfb_w's two launches: the partial products Wp = V^T C per row
+// slab, then W = op(T) sum Wp.
+void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
+                  double* Wp, int nsplit, hipStream_t s, const DevState* st);
+void launch_wsum(int j, int m, int c0, int n, const double* T, int trans, const double* Wp,
+                 double* W, int nsplit, hipStream_t s, const DevState* st);
 // Norm downdate of columns [j + fjb_cmp, n) (j, fjb_cmp from st; j0 <= j
 // sizes the grid).
 void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* work,
@@ -203,6 +218,8 @@ void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double*
                        hipStream_t s);
 int apply_perm_blocks(int m);
 void launch_clear_flag(DevState* st, int bits, hipStream_t s);
+// Clears the halt of a step whose exact panel the host queues next.
+void launch_resume(DevState* st, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
 
 }  // namespace qrdm
