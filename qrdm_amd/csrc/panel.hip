@@ -304,56 +304,79 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
   return nlim;
 }
 
-// X = R^-1 for a 64 x 64 upper triangular R with a non-zero diagonal
-// (R(i, l) = Rin[l][i] when trans, else Rin[i][l]); 256 threads.  The four
-// 16 x 16 diagonal blocks are inverted by the four waves (one lane per
-// column, back substitution), then the blocks above by distance d = 1..3:
-// X_ij = -X_ii sum_{k=i+1..j} R_ik X_kj.  Ends with a barrier.
-__device__ __forceinline__ void tri_inv_upper(const double (*Rin)[LDP], bool trans, double (*X)[LDP],
-                              double* rinv, double (*Y)[BLK][BLK + 1]) {
+// X := L^-1 and U^-1 of the pair held in one 64 x 64 matrix W (L unit lower
+// triangular strictly below the diagonal, U upper triangular on and above
+// it, non-zero diagonal), X in the same layout: L^-1 strictly below the
+// diagonal (its unit diagonal implied), U^-1 on and above; 256 threads, ends
+// with a barrier.  With W's strict lower part zero (L = I) this is U^-1 with
+// a zero lower part.  Both go by 16 x 16 blocks, L^-1 as (L^T)^-1 read and
+// stored transposed: the diagonal blocks by back substitution (wave w takes
+// block w of both, one lane per column: lanes 0-15 U, 16-31 L^T), then the
+// blocks at block distance d = 1, 2, 3 on the matrix cores, one block per
+// wave: X_ij = -X_ii Y with Y = sum_{k = i+1..j} R_ik X_kj (R = U or L^T).
+// Ys: one 16 x 16 scratch tile per wave.
+__device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[LDP], double* rinv,
+                                           double (*Ys)[BLK][BLK + 1]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  auto R = [&](int i, int l) -> double { return trans ? Rin[l][i] : Rin[i][l]; };
-  if (tid < KMAX) rinv[tid] = 1.0 / R(tid, tid);
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
-    if ((r >> 4) > (c >> 4)) X[r][c] = 0.0;
-  }
+  // R_m(i, l): m = 1 U, m = 0 L^T (unit diagonal); zero below the diagonal
+  auto Rm = [&](int m, int i, int l) -> double {
+    return m ? ((i <= l) ? W[i][l] : 0.0) : ((i < l) ? W[l][i] : (i == l ? 1.0 : 0.0));
+  };
+  // X_m(i, j) of the inverse of R_m, from X's storage
+  auto Xm = [&](int m, int i, int j) -> double {
+    return m ? ((i <= j) ? X[i][j] : 0.0) : ((i < j) ? X[j][i] : (i == j ? 1.0 : 0.0));
+  };
+  if (tid < KMAX) rinv[tid] = 1.0 / W[tid][tid];
   __syncthreads();
-  {
-    const int base = BLK * wave, lc = lane & 15;
+  if (lane < 32) {
+    const int m = (lane < 16) ? 1 : 0, c = lane & 15, base = BLK * wave;
     double x[BLK];
 #pragma unroll
     for (int i = BLK - 1; i >= 0; i--) {
-      double sacc = (i == lc) ? 1.0 : 0.0;
+      double sacc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int l = i + 1; l < BLK; l++) sacc = fma(-R(base + i, base + l), x[l], sacc);
-      x[i] = sacc * rinv[base + i];
+      for (int l = i + 1; l < BLK; l++) sacc = fma(-Rm(m, base + i, base + l), x[l], sacc);
+      x[i] = m ? sacc * rinv[base + i] : sacc;
     }
-    if (lane < BLK) {
 #pragma unroll
-      for (int i = 0; i < BLK; i++) X[base + i][base + lc] = x[i];
+    for (int i = 0; i < BLK; i++) {
+      if (m && i <= c) X[base + i][base + c] = x[i];
+      if (!m && i < c) X[base + c][base + i] = x[i];
     }
   }
   __syncthreads();
-  const int r = tid >> 4, c = tid & 15;
+  const int g = lane >> 4, li = lane & 15;
   for (int d = 1; d < 4; d++) {
-    for (int bi = 0; bi + d < 4; bi++) {
-      const int bj = bi + d;
-      double sacc = 0.0;
+    const int ntgt = 2 * (4 - d);
+    for (int t = wave; t < ntgt; t += 4) {
+      const int m = t & 1, bi = t >> 1, bj = bi + d;
+      dbl4p acc = {0.0, 0.0, 0.0, 0.0};
       for (int kb = bi + 1; kb <= bj; kb++) {
 #pragma unroll
-        for (int t = 0; t < BLK; t++)
-          sacc = fma(R(BLK * bi + r, BLK * kb + t), X[BLK * kb + t][BLK * bj + c], sacc);
+        for (int k0 = 0; k0 < BLK; k0 += 4) {
+          const double a = Rm(m, BLK * bi + li, BLK * kb + k0 + g);
+          const double b = Xm(m, BLK * kb + k0 + g, BLK * bj + li);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        }
       }
-      Y[bi][r][c] = sacc;
-    }
-    __syncthreads();
-    for (int bi = 0; bi + d < 4; bi++) {
-      const int bj = bi + d;
-      double sacc = 0.0;
 #pragma unroll
-      for (int t = 0; t < BLK; t++) sacc = fma(X[BLK * bi + r][BLK * bi + t], Y[bi][t][c], sacc);
-      X[BLK * bi + r][BLK * bj + c] = -sacc;
+      for (int v = 0; v < 4; v++) Ys[wave][g + 4 * v][li] = acc[v];
+      wave_lds_sync();
+      dbl4p acc2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k0 = 0; k0 < BLK; k0 += 4) {
+        const double a = Xm(m, BLK * bi + li, BLK * bi + k0 + g);
+        acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ys[wave][k0 + g][li], acc2, 0, 0, 0);
+      }
+      wave_lds_sync();  // Ys is reused by the wave's next block
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = BLK * bi + g + 4 * v, c = BLK * bj + li;
+        if (m)
+          X[r][c] = -acc2[v];
+        else
+          X[c][r] = -acc2[v];
+      }
     }
     __syncthreads();
   }
@@ -367,7 +390,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
                            double tau_dm, double* R1, double* R1inv, DevState* st) {
   __shared__ double S[KMAX][LDP];
   __shared__ double X[KMAX][LDP];
-  __shared__ double Y[3][BLK][BLK + 1];
+  __shared__ double Y[4][BLK][BLK + 1];
   __shared__ double rinv[KMAX], diag[KMAX];
   __shared__ int srk[KMAX];
   __shared__ int s_done, s_why;
@@ -427,7 +450,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
     S[r][c] = in ? v : (r == c ? 1.0 : 0.0);
   }
   __syncthreads();
-  tri_inv_upper(S, false, X, rinv, Y);
+  lu_inverse(S, X, rinv, Y);  // S's lower part is zero: X = R1^-1
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
     const int r = e >> 6, c = e & 63;
     R1inv[r + c * KMAX] = (r < k1 && c < k1) ? X[r][c] : 0.0;
@@ -557,7 +580,7 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   __shared__ double SA[KMAX][LDP];  // R^-1
   __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
   __shared__ double SX[KMAX][LDP];  // G2, F or R2, R1^-1, then U
-  __shared__ double Y[3][BLK][BLK + 1];
+  __shared__ double Y[4][BLK][BLK + 1];
   __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
   __shared__ int s_done, s_why;
   if (!step_runs(st, RUN_FAST)) return;
@@ -693,7 +716,7 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
     __syncthreads();
     if (tid >= k1 && tid < KMAX) SR[tid][tid] = 1.0;
     __syncthreads();
-    tri_inv_upper(SR, false, SA, rinv, Y);
+    lu_inverse(SR, SA, rinv, Y);  // R = R2 R1 is upper triangular: SA = R^-1
   }
   // 4. Q11 = P_top R^-1 -> SW
   load_slab(SW, A, lda, j, j, j + k1, k1);
@@ -712,51 +735,53 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
     }
     return;
   }
-  // 6. top block (R rows scaled by the signs, V1 below), tau; U -> SX
+  // 6. top block (R rows scaled by the signs, V1 below), tau; SX := the
+  // signed U (on and above the diagonal) and L (below), identity beyond k
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
+    const int r = e & 63, c = e >> 6;  // consecutive threads: consecutive rows
     const bool in = r < k && c < k;
     const double w = SW[r][c];
     if (in) A[(size_t)(j + c) * lda + j + r] = (r <= c) ? dsg[r] * SR[r][c] : w;
-    SX[r][c] = in ? (r < c ? dsg[c] * w : (r == c ? -tl[c] : 0.0)) : (r == c ? 1.0 : 0.0);
+    SX[r][c] = in ? (r < c ? dsg[c] * w : (r == c ? -tl[c] : w)) : (r == c ? 1.0 : 0.0);
   }
   if (tid < k) tau[tid] = tl[tid];
   __syncthreads();
-  // L^T (unit upper) -> SR
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
-    SR[r][c] = (r < c && c < k) ? SW[c][r] : (r == c ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  // 7. L^-T = (L^T)^-1 -> SW;  T = -U L^-T  (T V1^T = -U)
-  tri_inv_upper(SR, false, SW, rinv, Y);
-  mfma_64(acc, SX, false, SW);
+  // 7. SR := L^-1 (below the diagonal) and U^-1 (on and above)
+  lu_inverse(SX, SR, rinv, Y);
+  // 8. T = -U L^-T (T V1^T = -U) and M = R^-1 D U^-1, computed transposed so
+  // that consecutive lanes store consecutive rows: wave w takes columns
+  // 16w..16w+15 of T and M
   {
-    const int lane = tid & 63, wv = tid >> 6;
+    const int lane = tid & 63, wv = tid >> 6, li = lane & 15, g = lane >> 4;
+    dbl4p at[4], am[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++)
+    for (int u = 0; u < 4; u++) {
+      at[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+      am[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+    }
+    const int cc = 16 * wv + li;  // column of T / M (row of the transposed product)
+#pragma unroll 2
+    for (int k0 = 0; k0 < KMAX; k0 += 4) {
+      const int q = k0 + g;
+      // T^T(c, r) = sum_q L^-1(c, q) U(r, q);  M^T(c, r) = sum_q (D U^-1)(q, c) R^-1(r, q)
+      const double a_t = (q < cc) ? SR[cc][q] : (q == cc ? 1.0 : 0.0);
+      const double a_m = (q <= cc && q < k) ? dsg[q] * SR[q][cc] : 0.0;
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-        T_out[r + c * KMAX] = (r < k && c < k) ? -acc[u][v] : 0.0;
+      for (int u = 0; u < 4; u++) {
+        const int rr = 16 * u + li;
+        const double b_t = (q >= rr) ? SX[rr][q] : 0.0;
+        at[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_t, b_t, at[u], 0, 0, 0);
+        am[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_m, SA[rr][q], am[u], 0, 0, 0);
       }
-  }
-  // 8. U^-1 -> SR, rows scaled by D;  M = R^-1 D U^-1
-  tri_inv_upper(SX, false, SR, rinv, Y);
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
-    SR[r][c] *= (r < k) ? dsg[r] : 0.0;
-  }
-  __syncthreads();
-  mfma_64(acc, SA, false, SR);
-  {
-    const int lane = tid & 63, wv = tid >> 6;
+    }
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-        M_out[r + c * KMAX] = (r < k && c < k) ? acc[u][v] : 0.0;
+        const int c = 16 * wv + g + 4 * v, r = 16 * u + li;
+        const bool in = r < k && c < k;
+        T_out[r + c * KMAX] = in ? -at[u][v] : 0.0;
+        M_out[r + c * KMAX] = in ? am[u][v] : 0.0;
       }
   }
   if (tid == 0) {
