@@ -29,10 +29,11 @@ METRIC = ("DM-QR time & TF/s, 8192² FP64; mixed/learned vs plain at "
 FP64_MFMA_PEAK_TFS = 78.6
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_gemm1 launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
-# counter's reading for a plain copy of known size), averaged over the launches
-# of one factorisation of the workload (m, n, rank, nb); measured offline with
-# rocprofv3 --pmc, summary in profiles/k_gemm1_8192_pmc_traffic.txt.
-GEMM1_TRAFFIC = {(8192, 8192, 2048, 64): 260096442}
+# counter's reading for a transfer of known size in the same run: the initial
+# norm sweep reads the matrix once, the restore copy writes it once), averaged
+# over the launches of the workload (m, n, rank, nb); measured with
+# rocprofv3 --pmc, summary in profiles/k_gemm1_8192_pmc_traffic.md.
+GEMM1_TRAFFIC = {(8192, 8192, 2048, 64): 209483276}
 
 
 def gemm1_algorithmic_bytes(m, n, ncols):
