@@ -423,8 +423,16 @@ __global__ __launch_bounds__(64 * TOPK_WAVES) void k_topk(const double* __restri
     // wave w merges lists w, w + TOPK_WAVES, ... into its own
     v = -1.0;  // a sorted list of padding keys
     ix = 0x7fffffc0 + lane;
-    for (int b = wave; b < nblk; b += TOPK_WAVES)
-      wave_merge64(v, ix, ld_dev(tmp_val + b * KMAX + lane), ld_dev(tmp_idx + b * KMAX + lane));
+    // two lists per round, both loads in flight before the merges
+    for (int b = wave; b < nblk; b += 2 * TOPK_WAVES) {
+      const int b2 = b + TOPK_WAVES;
+      const double v1 = ld_dev(tmp_val + b * KMAX + lane);
+      const int i1 = ld_dev(tmp_idx + b * KMAX + lane);
+      const double v2 = (b2 < nblk) ? ld_dev(tmp_val + b2 * KMAX + lane) : -1.0;
+      const int i2 = (b2 < nblk) ? ld_dev(tmp_idx + b2 * KMAX + lane) : 0x7fffff80 + lane;
+      wave_merge64(v, ix, v1, i1);
+      if (b2 < nblk) wave_merge64(v, ix, v2, i2);
+    }
     lv[wave][lane] = v;
     li[wave][lane] = ix;
     __syncthreads();
@@ -587,10 +595,10 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 //    the candidate rank of each panel column into perm[4*KMAX..5*KMAX), and
 //    the panel form (Gram-based when fjb >= 2 and m - j >= fast_rows).
 // Run by one wave (threadIdx.x < 64); its LDS steps are ordered by
-// wave_lds_sync, so the rest of the workgroup may be elsewhere.  G is read
-// with device-coherent loads (the launch's other workgroups wrote it).
-__device__ void select_wave(const double* __restrict__ G, double delta, int m, int n,
-                            int fast_rows, int exact_queued, int* perm, DevState* st) {
+// wave_lds_sync, so the rest of the workgroup may be elsewhere.  Gs: the
+// candidate Gram in LDS, column-major with leading dimension KMAX + 1.
+__device__ void select_wave(const double* Gs, double delta, int m, int n, int fast_rows,
+                            int exact_queued, int* perm, DevState* st) {
   __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
   __shared__ int s_occ[KMAX], s_slot[KMAX];
@@ -609,7 +617,7 @@ __device__ void select_wave(const double* __restrict__ G, double delta, int m, i
     unsigned long long conf = 0ull;
 #pragma unroll 8
     for (int q = 0; q < KMAX; q++) {
-      const double c = fabs(ld_dev(G + t + q * KMAX) * s_r[q] * rt);  // G symmetric: G(q, t)
+      const double c = fabs(Gs[t + q * (KMAX + 1)] * s_r[q] * rt);  // G symmetric: G(q, t)
       conf |= (q < t && q < nc && c >= delta) ? (1ull << q) : 0ull;
     }
     for (int i = 1; i < nc; i++) {
@@ -718,6 +726,7 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
                                                        int exact_queued, int* perm,
                                                        unsigned* ticket, DevState* st) {
   __shared__ double part[4][64];
+  __shared__ double Gs[KMAX * (KMAX + 1)];
   __shared__ int s_last;
   if (dm_stopped(st)) return;
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -745,7 +754,19 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
   if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  if (threadIdx.x < 64) select_wave(G, delta, m, n, fast_rows, exact_queued, perm, st);
+  {
+    // G from the launch's workgroups into LDS, every load in flight at once
+    double gv[KMAX * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) gv[q] = ld_dev(G + threadIdx.x + 256 * q);
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e2 = threadIdx.x + 256 * q;
+      Gs[(e2 & 63) + (e2 >> 6) * (KMAX + 1)] = gv[q];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) select_wave(Gs, delta, m, n, fast_rows, exact_queued, perm, st);
 }
 
 void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,

@@ -60,6 +60,15 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
   return y;
 }
 
+// Workgroup barrier that orders LDS only: this file's one-workgroup stages
+// share data through LDS, never through global memory, so their global
+// stores may still be in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // 1/x for x in [1, 2]: the hardware estimate and two Newton steps.
 __device__ __forceinline__ double rcp_nr12(double x) {
   double y = __builtin_amdgcn_rcp(x);
@@ -151,7 +160,7 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
         *s_why = code;
       }
     }
-    __syncthreads();
+    lds_barrier();
     const int done = *s_done, code = *s_why;
     if (done < BLK || code) {
       *why = code;
@@ -174,9 +183,9 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
 #pragma unroll
       for (int i = 0; i < BLK; i++) S[bs + i][c] = x[i];
     }
-    __syncthreads();
+    lds_barrier();
     mfma_trailing(S, bs, lo, nlim, true);
-    __syncthreads();
+    lds_barrier();
   }
   *why = 0;
   return nlim;
@@ -253,7 +262,7 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
         *s_why = code;
       }
     }
-    __syncthreads();
+    lds_barrier();
     const int done = *s_done, code = *s_why;
     if (done < BLK || code) {
       *why = code;
@@ -296,9 +305,9 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
         for (int i = 0; i < BLK; i++) W[bs + i][c] = y[i];
       }
     }
-    __syncthreads();
+    lds_barrier();
     mfma_trailing(W, bs, lo, nlim, false);
-    __syncthreads();
+    lds_barrier();
   }
   *why = 0;
   return nlim;
@@ -308,13 +317,14 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
 // triangular strictly below the diagonal, U upper triangular on and above
 // it, non-zero diagonal), X in the same layout: L^-1 strictly below the
 // diagonal (its unit diagonal implied), U^-1 on and above; 256 threads, ends
-// with a barrier.  With W's strict lower part zero (L = I) this is U^-1 with
-// a zero lower part.  Both go by 16 x 16 blocks, L^-1 as (L^T)^-1 read and
+// with a barrier.  BOTH = false: U^-1 only (W's lower part is not read, X's
+// is set to zero).  Both go by 16 x 16 blocks, L^-1 as (L^T)^-1 read and
 // stored transposed: the diagonal blocks by back substitution (wave w takes
 // block w of both, one lane per column: lanes 0-15 U, 16-31 L^T), then the
 // blocks at block distance d = 1, 2, 3 on the matrix cores, one block per
 // wave: X_ij = -X_ii Y with Y = sum_{k = i+1..j} R_ik X_kj (R = U or L^T).
 // Ys: one 16 x 16 scratch tile per wave.
+template <bool BOTH = true>
 __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[LDP], double* rinv,
                                            double (*Ys)[BLK][BLK + 1]) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -327,8 +337,14 @@ __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[L
     return m ? ((i <= j) ? X[i][j] : 0.0) : ((i < j) ? X[j][i] : (i == j ? 1.0 : 0.0));
   };
   if (tid < KMAX) rinv[tid] = 1.0 / W[tid][tid];
-  __syncthreads();
-  if (lane < 32) {
+  if (!BOTH) {
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e & 63, c = e >> 6;
+      if (r > c) X[r][c] = 0.0;
+    }
+  }
+  lds_barrier();
+  if (lane < (BOTH ? 32 : 16)) {
     const int m = (lane < 16) ? 1 : 0, c = lane & 15, base = BLK * wave;
     double x[BLK];
 #pragma unroll
@@ -344,12 +360,12 @@ __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[L
       if (!m && i < c) X[base + c][base + i] = x[i];
     }
   }
-  __syncthreads();
+  lds_barrier();
   const int g = lane >> 4, li = lane & 15;
   for (int d = 1; d < 4; d++) {
-    const int ntgt = 2 * (4 - d);
+    const int ntgt = (BOTH ? 2 : 1) * (4 - d);
     for (int t = wave; t < ntgt; t += 4) {
-      const int m = t & 1, bi = t >> 1, bj = bi + d;
+      const int m = BOTH ? (t & 1) : 1, bi = BOTH ? (t >> 1) : t, bj = bi + d;
       dbl4p acc = {0.0, 0.0, 0.0, 0.0};
       for (int kb = bi + 1; kb <= bj; kb++) {
 #pragma unroll
@@ -378,7 +394,7 @@ __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[L
           X[c][r] = -acc2[v];
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 
@@ -402,7 +418,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
     srk[tid] = rk;
     diag[tid] = tid < fjb ? Gc[rk * (KMAX + 1)] : 0.0;
   }
-  __syncthreads();
+  lds_barrier();
   {
     double gv[KMAX * KMAX / 256];
 #pragma unroll
@@ -423,7 +439,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * sqrt(g00) : 5e-14;
   const double thr2 = 0.25 * thres * thres;
   const bool scale_ok = g00 > 1e-280 && g00 < 1e280 && maxd < 1e280;
-  __syncthreads();
+  lds_barrier();
   int why = 0;
   const int k1 = chol_blocked(
       S, fjb,
@@ -449,8 +465,8 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
     R1[r + c * KMAX] = v;
     S[r][c] = in ? v : (r == c ? 1.0 : 0.0);
   }
-  __syncthreads();
-  lu_inverse(S, X, rinv, Y);  // S's lower part is zero: X = R1^-1
+  lds_barrier();
+  lu_inverse<false>(S, X, rinv, Y);  // X = R1^-1 (upper part)
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
     const int r = e >> 6, c = e & 63;
     R1inv[r + c * KMAX] = (r < k1 && c < k1) ? X[r][c] : 0.0;
@@ -553,12 +569,12 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
   const int rs = j + blockIdx.x * PR_ROWS;
   load_slab(Ps, A, lda, j, rs, m, k1);
   load_mat(Ri, R1inv);
-  __syncthreads();
+  lds_barrier();
   dbl4p acc[4];
   mfma_64(acc, Ps, false, Ri);
-  __syncthreads();
+  lds_barrier();
   mfma_store(Ps, acc);
-  __syncthreads();
+  lds_barrier();
   mfma_64(acc, Ps, true, Ps);  // Q^T Q
   const int lane = tid & 63, wv = tid >> 6;
   double* out = partial + (size_t)blockIdx.x * KMAX * KMAX;
@@ -605,7 +621,7 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       SR[r][c] = gr[q];
     }
   }
-  __syncthreads();
+  lds_barrier();
   // 1-3. R = R2 R1 with R2^T R2 = G2 and R^-1 (into SR and SA).  After the
   // first pass G2 = I + E with E tiny: then R2 = I + F, F upper triangular
   // with F + F^T + F^T F = E, is the fixed point of F = triu_half(E - F^T F)
@@ -627,9 +643,9 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
     em = fmax(em, __shfl_xor(em, 2, 64));
     em = fmax(em, __shfl_xor(em, 1, 64));
     if ((tid & 63) == 0) rinv[tid >> 6] = em;
-    __syncthreads();
+    lds_barrier();
     emax = fmax(fmax(rinv[0], rinv[1]), fmax(rinv[2], rinv[3]));
-    __syncthreads();
+    lds_barrier();
   }
   dbl4p acc[4];
   int why = 0;
@@ -644,10 +660,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       SW[r][c] = ev;
       SX[r][c] = (r < c) ? ev : (r == c ? 0.5 * ev : 0.0);
     }
-    __syncthreads();
+    lds_barrier();
     for (int ps = 0; ps < npass; ps++) {
       mfma_64(acc, SX, true, SX);  // F^T F
-      __syncthreads();
+      lds_barrier();
       const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
       for (int u = 0; u < 4; u++)
@@ -657,11 +673,11 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
           const double x = SW[r][c] - acc[u][v];
           SX[r][c] = (r < c) ? x : (r == c ? 0.5 * x : 0.0);
         }
-      __syncthreads();
+      lds_barrier();
     }
     // R = R1 + F R1
     mfma_64(acc, SX, false, SR);
-    __syncthreads();
+    lds_barrier();
     {
       const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
@@ -674,10 +690,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       const int r = e >> 6, c = e & 63;
       SW[r][c] = (r == c ? 1.0 : 0.0) - SX[r][c];
     }
-    __syncthreads();
+    lds_barrier();
     for (int ps = 0; ps < npass; ps++) {
       mfma_64(acc, SX, false, SW);
-      __syncthreads();
+      lds_barrier();
       const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll
       for (int u = 0; u < 4; u++)
@@ -686,14 +702,14 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
           const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
           SW[r][c] = (r == c ? 1.0 : 0.0) - acc[u][v];
         }
-      __syncthreads();
+      lds_barrier();
     }
     // R^-1 = R1^-1 R2^-1
     load_mat(SX, R1inv);
-    __syncthreads();
+    lds_barrier();
     mfma_64(acc, SX, false, SW);
     mfma_store(SA, acc);
-    __syncthreads();
+    lds_barrier();
   } else {
     chol_blocked(
         SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv,
@@ -709,22 +725,22 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       const int r = e >> 6, c = e & 63;
       if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
     }
-    __syncthreads();
+    lds_barrier();
     mfma_64(acc, SX, false, SR);  // R = R2 R1, identity-padded for the inverse
-    __syncthreads();
+    lds_barrier();
     mfma_store(SR, acc);
-    __syncthreads();
+    lds_barrier();
     if (tid >= k1 && tid < KMAX) SR[tid][tid] = 1.0;
-    __syncthreads();
-    lu_inverse(SR, SA, rinv, Y);  // R = R2 R1 is upper triangular: SA = R^-1
+    lds_barrier();
+    lu_inverse<false>(SR, SA, rinv, Y);  // SA = R^-1 (upper part)
   }
   // 4. Q11 = P_top R^-1 -> SW
   load_slab(SW, A, lda, j, j, j + k1, k1);
-  __syncthreads();
+  lds_barrier();
   mfma_64(acc, SW, false, SA);
-  __syncthreads();
+  lds_barrier();
   mfma_store(SW, acc);
-  __syncthreads();
+  lds_barrier();
   // 5. LU of Q11 D - I with dlarfg's signs and the truncation test
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SR[0][0] : 5e-14;
   const int k = lu_signed_blocked(SW, k1, SR, thres, dsg, tl, scv, &s_done, &s_why, &why);
@@ -745,7 +761,7 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
     SX[r][c] = in ? (r < c ? dsg[c] * w : (r == c ? -tl[c] : w)) : (r == c ? 1.0 : 0.0);
   }
   if (tid < k) tau[tid] = tl[tid];
-  __syncthreads();
+  lds_barrier();
   // 7. SR := L^-1 (below the diagonal) and U^-1 (on and above)
   lu_inverse(SX, SR, rinv, Y);
   // 8. T = -U L^-T (T V1^T = -U) and M = R^-1 D U^-1, computed transposed so
