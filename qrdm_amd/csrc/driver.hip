@@ -138,7 +138,7 @@ struct Buffers {
   int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
   int* jp;     // n: jpvt on the device during the DM loop
   int* ncols;  // n: fjb_cmp of each DM step
-  unsigned* tickets;  // counters (zero between uses); tickets[0]: the top-k stage
+  unsigned* tickets;  // launch tickets (zero between uses): [0] top-k, [1] selection, [2] reconstruction
   int nsplit;
 };
 
@@ -503,7 +503,7 @@ void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buf
     launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
     launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
     launch_larft(B.S, tau + j, B.st, B.T, s);
-    launch_larfb(A, lda, j, m, j + kb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, B.tickets, s);
+    launch_larfb(A, lda, j, m, j + kb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
   }
 }
 
@@ -531,7 +531,7 @@ int run_ormqr(int trans, int m, int n, int k, const double* A, int lda, const do
     launch_build_v(A, lda, j, m, B.ldv, B.st, B.V, B.ldv, s);
     launch_gram(B.V, B.ldv, j, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
     launch_larft(B.S, tau + j, B.st, B.T, s);
-    launch_larfb(C, ldc, j, m, 0, n, B.V, B.ldv, B.T, trans, B.Wp, B.W, B.nsplit, B.tickets, s);
+    launch_larfb(C, ldc, j, m, 0, n, B.V, B.ldv, B.T, trans, B.Wp, B.W, B.nsplit, s);
   }
   HIPCHK(hipStreamSynchronize(s));
   return 0;

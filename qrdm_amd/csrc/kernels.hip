@@ -1420,9 +1420,8 @@ void launch_wsum(int j, int m, int c0, int n, const double* T, int trans, const 
 }
 
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                    const double* T, int trans, double* Wp, double* W, int nsplit,
-                    unsigned* tickets, hipStream_t s, const DevState* st) {
-  (void)tickets;
+                    const double* T, int trans, double* Wp, double* W, int nsplit, hipStream_t s,
+                    const DevState* st) {
   launch_gemm1(A, lda, j, m, c0, n, V, ldv, Wp, nsplit, s, st);
   launch_wsum(j, m, c0, n, T, trans, Wp, W, nsplit, s, st);
 }
@@ -1437,9 +1436,8 @@ void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const d
 }
 
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                  const double* T, int trans, double* Wp, double* W, int nsplit,
-                  unsigned* tickets, hipStream_t s) {
-  launch_larfb_w(A, lda, j, m, c0, n, V, ldv, T, trans, Wp, W, nsplit, tickets, s, nullptr);
+                  const double* T, int trans, double* Wp, double* W, int nsplit, hipStream_t s) {
+  launch_larfb_w(A, lda, j, m, c0, n, V, ldv, T, trans, Wp, W, nsplit, s, nullptr);
   launch_larfb_apply(A, lda, j, m, c0, n, V, ldv, W, s, nullptr);
 }
 
@@ -1543,23 +1541,13 @@ void launch_set_kk(DevState* st, int kk, hipStream_t s) {
   hipLaunchKernelGGL(k_set_kk, dim3(1), dim3(1), 0, s, st, kk);
 }
 
-__global__ void k_clear_flag(DevState* st, int bits) { st->flags &= ~bits; }
-
 __global__ void k_resume(DevState* st) { st->halt = 0; }
 
 void launch_resume(DevState* st, hipStream_t s) {
   hipLaunchKernelGGL(k_resume, dim3(1), dim3(1), 0, s, st);
 }
 
-void launch_clear_flag(DevState* st, int bits, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, st, bits);
-}
 
-void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s,
-                        const DevState* st, int mode) {
-  hipLaunchKernelGGL(k_gram_reduce, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
-                     st, mode);
-}
 
 // Swap two whole columns (cblas_dswap of the fixed-column pre-pass,
 // src/dgeqrdm_work.c:599).

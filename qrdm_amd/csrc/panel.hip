@@ -5,35 +5,39 @@
 // and its top fjb x fjb block, so it is computed as:
 //   G   = P^T P: the DM step's candidate Gram (same columns, same rows j..m),
 //         gathered in panel order -- no pass over P;
-//   k_chol1  R1 = chol(G) and R1^-1 (one workgroup); the factorisation stops
-//         at a column whose pivot proves the DM truncation test fires there;
-//   pass 2   G2 = (P R1^-1)^T (P R1^-1): CholeskyQR2 re-orthogonalisation
+//   chol1_body  R1 = chol(G) and R1^-1 (one workgroup, in k_perm_chol1
+//         beside the column moves); the factorisation stops at a column
+//         whose pivot proves the DM truncation test fires there;
+//   k_qgram  G2 = (P R1^-1)^T (P R1^-1): CholeskyQR2 re-orthogonalisation
 //         (matrix cores, 64-row slabs);
-//   k_recon  R = chol(G2) R1; Q11 = P_top R^-1; LU without pivoting of
-//         Q11 D - I with the sign D chosen per column as dlarfg's
-//         beta = -sign(alpha) ||.|| does (Householder reconstruction,
-//         Ballard et al., "Reconstructing Householder vectors from
-//         tall-skinny QR", 2014): L = V1, U = -T V1^T, diag(U) = -tau;
-//         T from T V1^T = -U; M = R^-1 D U^-1.  The DM truncation test of
-//         column i uses ||x_i|| = R_ii sqrt(1 - a_i^2), a_i = the LU's
-//         unsigned pivot, against thres = tau_dm |R(0,0)| (or 5e-14), as
-//         dlarfg_mia does (src/dlarfg.c:126-133, src/dgeqr2.c:176-177);
-//   pass 3   V2 = P_bottom M (matrix cores), written into the panel and into
+//   recon_body  (the last workgroup of G2's reduce, k_reduce_recon)
+//         R = R2 R1 with R2^T R2 = G2; Q11 = P_top R^-1; LU without
+//         pivoting of Q11 D - I with the sign D chosen per column as
+//         dlarfg's beta = -sign(alpha) ||.|| does (Householder
+//         reconstruction, Ballard et al., "Reconstructing Householder
+//         vectors from tall-skinny QR", 2014): L = V1, U = -T V1^T,
+//         diag(U) = -tau; T from T V1^T = -U; M = R^-1 D U^-1.  The DM
+//         truncation test of column i uses ||x_i|| = R_ii sqrt(1 - a_i^2),
+//         a_i = the LU's unsigned pivot, against thres = tau_dm |R(0,0)|
+//         (or 5e-14), as dlarfg_mia does (src/dlarfg.c:126-133,
+//         src/dgeqr2.c:176-177);
+//   k_v2     V2 = P_bottom M (matrix cores), written into the panel and into
 //         the dense V the trailing update reads.
 // Columns the truncation rejects keep their original values; the caller's
 // trailing update then starts at j + fjb_cmp and applies the kept reflectors
 // to them, as dlarf_ did inside the reference's panel.
 //
 // Any degenerate case (a pivot that does not resolve, a reflector close to
-// tau = 2 or tau = 0, non-finite values) sets DevState::flags bit 1 before
-// anything is written, and the caller runs the exact panel instead.
+// tau = 2 or tau = 0, non-finite values) sets DevState::flags bit 1 and the
+// halt before anything is written, and the step continues with the exact
+// panel.
 //
-// The one-workgroup kernels (k_chol1, k_recon) keep 64 x 64 matrices in LDS
-// and work in 16-column blocks: wave 0 factors a diagonal block with one lane
-// per row, taking the pivot and the pivot row from lane i with readlane (no
-// barrier inside the block; each column's reciprocal or inverse square root
-// is a hardware estimate refined by Newton steps, and the column's results
-// are computed before its stop test, which only decides what is kept); all
+// The one-workgroup stages keep 64 x 64 matrices in LDS and work in
+// 16-column blocks: wave 0 factors a diagonal block with one lane per row,
+// taking the pivot and the pivot row from lane i with readlane (no barrier
+// inside the block; each column's reciprocal or inverse square root is a
+// hardware estimate refined by Newton steps, and the column's results are
+// computed before its stop test, which only decides what is kept); all
 // threads then solve the block row or column (one lane per row or column,
 // independent), and the matrix cores update the trailing matrix.  Three
 // barriers per block instead of one per column.
@@ -524,6 +528,69 @@ __device__ __forceinline__ void mfma_store(double (*S)[LDP], const dbl4p (&acc)[
     for (int v = 0; v < 4; v++) S[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] = acc[u][v];
 }
 
+// 64 x 64 products with triangular factors on the matrix cores, as jobs of
+// 16 x 16 output tiles (ti, tj) over 16-deep k blocks k0..k1, balanced over
+// the four waves (five k blocks each for the triangular kinds):
+//   TRI_UU  upper x upper (the result is upper): tiles ti <= tj, k blocks ti..tj;
+//   TRI_LU  lower x upper, the upper half of the result: ti <= tj, k blocks 0..ti;
+//   TRI_FU  full x upper: every tile, k blocks 0..tj (wave w: tile row w).
+enum { TRI_UU = 0, TRI_LU = 1, TRI_FU = 2 };
+
+struct TileJob {
+  int ti, tj, k0, k1;
+};
+
+__device__ __forceinline__ int tri_jobs(int kind, int wave, TileJob (&jb)[4]) {
+  if (kind == TRI_FU) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) jb[t] = TileJob{wave, t, 0, t};
+    return 4;
+  }
+  if (kind == TRI_UU) {
+    switch (wave) {
+      case 0: jb[0] = TileJob{0, 3, 0, 3}; jb[1] = TileJob{0, 0, 0, 0}; return 2;
+      case 1: jb[0] = TileJob{0, 2, 0, 2}; jb[1] = TileJob{0, 1, 0, 1}; return 2;
+      case 2: jb[0] = TileJob{1, 3, 1, 3}; jb[1] = TileJob{1, 2, 1, 2}; return 2;
+      default:
+        jb[0] = TileJob{2, 3, 2, 3}; jb[1] = TileJob{1, 1, 1, 1};
+        jb[2] = TileJob{2, 2, 2, 2}; jb[3] = TileJob{3, 3, 3, 3}; return 4;
+    }
+  }
+  switch (wave) {  // TRI_LU
+    case 0: jb[0] = TileJob{3, 3, 0, 3}; jb[1] = TileJob{0, 0, 0, 0}; return 2;
+    case 1: jb[0] = TileJob{2, 3, 0, 2}; jb[1] = TileJob{1, 1, 0, 1}; return 2;
+    case 2: jb[0] = TileJob{2, 2, 0, 2}; jb[1] = TileJob{1, 2, 0, 1}; return 2;
+    default:
+      jb[0] = TileJob{1, 3, 0, 1}; jb[1] = TileJob{0, 1, 0, 0};
+      jb[2] = TileJob{0, 2, 0, 0}; jb[3] = TileJob{0, 3, 0, 0}; return 4;
+  }
+}
+
+// acc[t] = the product's tile jb[t] (fa(row, q), fb(q, col) element access);
+// acc[t][v] holds row 16 ti + lane/16 + 4v, column 16 tj + lane%16.
+template <class FA, class FB>
+__device__ __forceinline__ int tri_product(int kind, FA fa, FB fb, TileJob (&jb)[4],
+                                           dbl4p (&acc)[4]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const int nj = tri_jobs(kind, wave, jb);
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    acc[t] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+    if (t < nj) {
+      for (int kb = jb[t].k0; kb <= jb[t].k1; kb++) {
+#pragma unroll
+        for (int k4 = 0; k4 < BLK; k4 += 4) {
+          const int q = BLK * kb + k4 + g;
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa(BLK * jb[t].ti + li, q),
+                                                        fb(q, BLK * jb[t].tj + li), acc[t], 0, 0,
+                                                        0);
+        }
+      }
+    }
+  }
+  return nj;
+}
+
 // Ps[r][c] = A(rs + r, j + c) for c < ncol and rows < m, else 0 (a 64-row
 // slab of the panel); 256 threads, all loads in flight before the LDS writes.
 __device__ __forceinline__ void load_slab(double (*Ps)[LDP], const double* __restrict__ A,
@@ -661,54 +728,67 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       SX[r][c] = (r < c) ? ev : (r == c ? 0.5 * ev : 0.0);
     }
     lds_barrier();
+    TileJob jb[4];
+    const int lane = tid & 63, g = lane >> 4, li = lane & 15;
     for (int ps = 0; ps < npass; ps++) {
-      mfma_64(acc, SX, true, SX);  // F^T F
+      // F := triu_half(E - F^T F): the upper half of F^T F (F upper)
+      const int nj = tri_product(
+          TRI_LU, [&](int r, int q) { return SX[q][r]; }, [&](int q, int c) { return SX[q][c]; },
+          jb, acc);
       lds_barrier();
-      const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int t = 0; t < nj; t++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-          const double x = SW[r][c] - acc[u][v];
+          const int r = BLK * jb[t].ti + g + 4 * v, c = BLK * jb[t].tj + li;
+          const double x = SW[r][c] - acc[t][v];
           SX[r][c] = (r < c) ? x : (r == c ? 0.5 * x : 0.0);
         }
       lds_barrier();
     }
-    // R = R1 + F R1
-    mfma_64(acc, SX, false, SR);
-    lds_barrier();
+    // R = R1 + F R1 (both upper)
     {
-      const int lane = tid & 63, wv = tid >> 6;
+      const int nj = tri_product(
+          TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SR[q][c]; },
+          jb, acc);
+      lds_barrier();
+      for (int t = 0; t < nj; t++)
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++) SR[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] += acc[u][v];
+        for (int v = 0; v < 4; v++) SR[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] += acc[t][v];
     }
-    // R2^-1 = I - F (I - F (...)) into SW
+    // R2^-1 = I - F (I - F (...)) into SW (upper)
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
       const int r = e >> 6, c = e & 63;
       SW[r][c] = (r == c ? 1.0 : 0.0) - SX[r][c];
     }
     lds_barrier();
     for (int ps = 0; ps < npass; ps++) {
-      mfma_64(acc, SX, false, SW);
+      const int nj = tri_product(
+          TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SW[q][c]; },
+          jb, acc);
       lds_barrier();
-      const int lane = tid & 63, wv = tid >> 6;
-#pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int t = 0; t < nj; t++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-          SW[r][c] = (r == c ? 1.0 : 0.0) - acc[u][v];
+          const int r = BLK * jb[t].ti + g + 4 * v, c = BLK * jb[t].tj + li;
+          SW[r][c] = (r == c ? 1.0 : 0.0) - acc[t][v];
         }
       lds_barrier();
     }
-    // R^-1 = R1^-1 R2^-1
+    // R^-1 = R1^-1 R2^-1 (both upper) into SA, zero below the diagonal blocks
     load_mat(SX, R1inv);
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e & 63, c = e >> 6;
+      if ((r >> 4) > (c >> 4)) SA[r][c] = 0.0;
+    }
     lds_barrier();
-    mfma_64(acc, SX, false, SW);
-    mfma_store(SA, acc);
+    {
+      const int nj = tri_product(
+          TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SW[q][c]; },
+          jb, acc);
+      for (int t = 0; t < nj; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) SA[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] = acc[t][v];
+    }
     lds_barrier();
   } else {
     chol_blocked(
@@ -737,9 +817,18 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   // 4. Q11 = P_top R^-1 -> SW
   load_slab(SW, A, lda, j, j, j + k1, k1);
   lds_barrier();
-  mfma_64(acc, SW, false, SA);
-  lds_barrier();
-  mfma_store(SW, acc);
+  {
+    TileJob jq[4];
+    const int lane = tid & 63, g = lane >> 4, li = lane & 15;
+    tri_product(
+        TRI_FU, [&](int r, int q) { return SW[r][q]; }, [&](int q, int c) { return SA[q][c]; }, jq,
+        acc);
+    lds_barrier();
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) SW[BLK * jq[t].ti + g + 4 * v][BLK * jq[t].tj + li] = acc[t][v];
+  }
   lds_barrier();
   // 5. LU of Q11 D - I with dlarfg's signs and the truncation test
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SR[0][0] : 5e-14;
@@ -764,41 +853,51 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   lds_barrier();
   // 7. SR := L^-1 (below the diagonal) and U^-1 (on and above)
   lu_inverse(SX, SR, rinv, Y);
-  // 8. T = -U L^-T (T V1^T = -U) and M = R^-1 D U^-1, computed transposed so
-  // that consecutive lanes store consecutive rows: wave w takes columns
-  // 16w..16w+15 of T and M
+  // 8. T = -U L^-T (T V1^T = -U) and M = R^-1 D U^-1, both upper triangular,
+  // computed transposed (lower) so that consecutive lanes store consecutive
+  // rows: the TRI_UU jobs with the tile indices swapped; the tiles below the
+  // diagonal blocks of T and M are zero
   {
     const int lane = tid & 63, wv = tid >> 6, li = lane & 15, g = lane >> 4;
+    TileJob jt[4];
+    const int nj = tri_jobs(TRI_UU, wv, jt);
     dbl4p at[4], am[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      at[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
-      am[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
-    }
-    const int cc = 16 * wv + li;  // column of T / M (row of the transposed product)
-#pragma unroll 2
-    for (int k0 = 0; k0 < KMAX; k0 += 4) {
-      const int q = k0 + g;
-      // T^T(c, r) = sum_q L^-1(c, q) U(r, q);  M^T(c, r) = sum_q (D U^-1)(q, c) R^-1(r, q)
-      const double a_t = (q < cc) ? SR[cc][q] : (q == cc ? 1.0 : 0.0);
-      const double a_m = (q <= cc && q < k) ? dsg[q] * SR[q][cc] : 0.0;
+    for (int t = 0; t < 4; t++) {
+      at[t] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+      am[t] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+      if (t < nj) {
+        const int cc = BLK * jt[t].tj + li;  // column of T / M (row of the transposed product)
+        const int rr = BLK * jt[t].ti + li;  // row of T / M
+        for (int kb = jt[t].k0; kb <= jt[t].k1; kb++) {
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int rr = 16 * u + li;
-        const double b_t = (q >= rr) ? SX[rr][q] : 0.0;
-        at[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_t, b_t, at[u], 0, 0, 0);
-        am[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_m, SA[rr][q], am[u], 0, 0, 0);
+          for (int k4 = 0; k4 < BLK; k4 += 4) {
+            const int q = BLK * kb + k4 + g;
+            // T^T(c, r) = sum_q L^-1(c, q) U(r, q);  M^T(c, r) = sum_q (D U^-1)(q, c) R^-1(r, q)
+            const double a_t = (q < cc) ? SR[cc][q] : (q == cc ? 1.0 : 0.0);
+            const double a_m = (q <= cc && q < k) ? dsg[q] * SR[q][cc] : 0.0;
+            const double b_t = (q >= rr) ? SX[rr][q] : 0.0;
+            at[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_t, b_t, at[t], 0, 0, 0);
+            am[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_m, SA[rr][q], am[t], 0, 0, 0);
+          }
+        }
+        // tile rows = columns c of T / M, tile columns = rows r
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = BLK * jt[t].tj + g + 4 * v, r = BLK * jt[t].ti + li;
+          const bool in = r < k && c < k;
+          T_out[r + c * KMAX] = in ? -at[t][v] : 0.0;
+          M_out[r + c * KMAX] = in ? am[t][v] : 0.0;
+        }
       }
     }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int c = 16 * wv + g + 4 * v, r = 16 * u + li;
-        const bool in = r < k && c < k;
-        T_out[r + c * KMAX] = in ? -at[u][v] : 0.0;
-        M_out[r + c * KMAX] = in ? am[u][v] : 0.0;
+    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+      const int r = e & 63, c = e >> 6;
+      if ((r >> 4) > (c >> 4)) {
+        T_out[r + c * KMAX] = 0.0;
+        M_out[r + c * KMAX] = 0.0;
       }
+    }
   }
   if (tid == 0) {
     st->fjb_cmp = k;

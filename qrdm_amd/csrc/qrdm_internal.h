@@ -172,13 +172,12 @@ void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState*
 // trans = 1: C := (I - V T V^T)^T C (dgeqrdm's trailing update);
 // trans = 0: C := (I - V T V^T) C.
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                  const double* T, int trans, double* Wp, double* W, int nsplit,
-                  unsigned* tickets, hipStream_t s);
+                  const double* T, int trans, double* Wp, double* W, int nsplit, hipStream_t s);
 // The two halves of launch_larfb: W = op(T) V^T C, then C -= V W.  With st
 // (the DM loop) j and c0 come from st and the host values are lower bounds.
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
-                    const double* T, int trans, double* Wp, double* W, int nsplit,
-                    unsigned* tickets, hipStream_t s, const DevState* st = nullptr);
+                    const double* T, int trans, double* Wp, double* W, int nsplit, hipStream_t s,
+                    const DevState* st = nullptr);
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                         const double* W, hipStream_t s, const DevState* st = nullptr);
 // launch_larfb_w's two launches: the partial products Wp = V^T C per row
@@ -201,8 +200,6 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
 void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
-void launch_gram_reduce(const double* partial, int nslab, double* G, hipStream_t s,
-                        const DevState* st = nullptr, int mode = RUN_ALWAYS);
 // Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST
 // kernels: j, fjb from st.  launch_perm_chol1 moves the step's columns (as
 // launch_apply_perm) and factors the panel's Gram, gathered from the candidate
@@ -217,7 +214,6 @@ void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double*
                        double* M, double* V, int ldv, unsigned* ticket, DevState* st,
                        hipStream_t s);
 int apply_perm_blocks(int m);
-void launch_clear_flag(DevState* st, int bits, hipStream_t s);
 // Clears the halt of a step whose exact panel the host queues next.
 void launch_resume(DevState* st, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
