@@ -103,6 +103,47 @@ def cpu_baseline(n_sample=8192, rank_sample=2048):
             "seconds": round(t_dm, 3), "dgeqp3_seconds": round(t_qp3, 3)}
 
 
+def backward_error(torch, qrdm_amd, work, pristine, tau, jp, stream):
+    """||A P - Q R||_F / ||A||_F of the factorisation held in work (R and the
+    reflectors, column-major m x n in an (n, m) tensor), Q R formed by the
+    library's dormqr on the device."""
+    import ctypes
+    n, m = work.shape
+    R = torch.triu(work.T)
+    qr = R.T.contiguous()
+    info = qrdm_amd.lib().qrdm_dormqr_dev(0, m, n, min(m, n), ctypes.c_void_p(work.data_ptr()),
+                                          m, ctypes.c_void_p(tau.data_ptr()),
+                                          ctypes.c_void_p(qr.data_ptr()), m,
+                                          ctypes.c_void_p(stream))
+    if info != 0:
+        raise RuntimeError("dormqr_dev info %d" % info)
+    A = pristine.T
+    perm = torch.from_numpy(jp.astype("int64") - 1).to(work.device)
+    return (torch.linalg.norm(qr.T - A[:, perm]) / torch.linalg.norm(A)).item()
+
+
+def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, stream, ms_plain):
+    """The plain FP64 path beside the FP32 and split-BF16x3 trailing updates
+    on the same matrix: time per factorisation (same timing as the headline,
+    no events) and the backward error of one factorisation of each."""
+    m, n = args.m, args.n
+    out = {}
+    for form in ("fp64", "fp32", "bf16x3"):
+        ms_f = ms_plain
+        if form != "fp64":
+            timed(1, form)  # warm-up of the form's kernels
+            ms_f, _ = timed(args.steps, form)
+        # one more factorisation of this form, left in work, for its backward error
+        timed(1, form)
+        be = backward_error(torch, qrdm_amd, work, pristine, tau, last_jp, stream)
+        out[form] = {"ms_per_step": round(ms_f, 3),
+                     "tfs": round(flops(m, n) / (ms_f / 1000.0) / 1e12, 4),
+                     "speedup_vs_fp64": round(ms_plain / ms_f, 4),
+                     "backward_error": float("%.4g" % be),
+                     "meets_1e-12": bool(be <= 1e-12)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,12 +172,16 @@ def main():
     thres = np.array([0.9, 0.15, 0.0])
     stream = torch.cuda.current_stream().cuda_stream
 
-    def one_step():
+    last_jp = np.zeros(n, np.int32)
+
+    def one_step(trailing="fp64"):
         jp = np.zeros(n, np.int32)
         nc = np.zeros(n, np.int32)
-        info, _ = qrdm_amd.dgeqrdm_dev(work, m, tau, jp, nc, thres, nb=args.nb, stream=stream)
+        info, _ = qrdm_amd.dgeqrdm_dev(work, m, tau, jp, nc, thres, nb=args.nb, stream=stream,
+                                       trailing=trailing)
         if info != 0:
             raise RuntimeError("dgeqrdm_dev info %d: %s" % (info, qrdm_amd.last_error()))
+        last_jp[:] = jp
         return nc
 
     for _ in range(args.warmup):
@@ -145,13 +190,13 @@ def main():
         one_step()
     torch.cuda.synchronize()
 
-    def timed(steps):
+    def timed(steps, trailing="fp64"):
         total, nc = 0.0, None
         for _ in range(steps):
             work.copy_(pristine)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            nc = one_step()
+            nc = one_step(trailing)
             torch.cuda.synchronize()
             total += time.perf_counter() - t0
         return 1000.0 * total / steps, nc
@@ -217,6 +262,8 @@ def main():
                        "dm_steps": nsteps_dm,
                        "kernel_classes_one_factorisation": others},
             "roofline": roofline, "roofline_apply": roofline_apply}
+    line["forms"] = trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp,
+                                   stream, ms)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

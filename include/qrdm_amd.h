@@ -53,6 +53,20 @@ int dgeqrdm(int matrix_layout, int m, int n, double *a, int lda, int *jpvt, doub
 int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
                      double *d_tau, int *ncols, const double *thres, int nb, void *stream);
 
+/* Precision of the trailing update's two products (qrdm_opts.trailing). */
+#define QRDM_TRAILING_FP64 0   /* FP64 matrix cores (the plain path) */
+#define QRDM_TRAILING_FP32 1   /* FP32 matrix cores, FP32 accumulation */
+#define QRDM_TRAILING_BF16X3 2 /* split BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+
+/* Options of qrdm_dgeqrdm_dev_opt; a NULL pointer means all defaults. */
+typedef struct {
+  int trailing;       /* QRDM_TRAILING_*: only C := C - V T^T V^T C changes
+                         precision (C itself stays FP64 in memory); the panel,
+                         the norm downdate/recompute and the cosine test stay
+                         FP64 */
+  const void *policy; /* reserved, must be NULL */
+} qrdm_opts;
+
 /* Per-step record of the DM loop (debugging and step-by-step parity). */
 typedef struct {
   int j;         /* 0-based first column of the step */
@@ -75,6 +89,17 @@ typedef struct {
 int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
                            double *d_tau, int *ncols, const double *thres, int nb, void *stream,
                            qrdm_step *trace, int ntrace, int *nsteps);
+
+/*
+ * qrdm_dgeqrdm_dev_trace with options (opts may be NULL).  No counterpart in
+ * the reference, which has only the FP64 path: the reduced-precision
+ * trailing updates trade the 1e-12 agreement with the FP64 path for speed
+ * (backward error about the FP32 unit roundoff times the matrix's norm).
+ */
+int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
+                         double *d_tau, int *ncols, const double *thres, int nb,
+                         const qrdm_opts *opts, void *stream, qrdm_step *trace, int ntrace,
+                         int *nsteps);
 
 /*
  * Unpivoted Householder QR on the device (LAPACK dgeqrf semantics; the

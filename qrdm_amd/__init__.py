@@ -18,6 +18,14 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
 
+class Opts(ctypes.Structure):
+    """qrdm_opts of include/qrdm_amd.h."""
+    _fields_ = [("trailing", ctypes.c_int), ("policy", ctypes.c_void_p)]
+
+
+TRAILING = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+
+
 class Step(ctypes.Structure):
     """qrdm_step of include/qrdm_amd.h."""
     _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
@@ -26,7 +34,8 @@ class Step(ctypes.Structure):
 
 
 # Every entry point include/qrdm_amd.h declares.
-EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrf_dev",
+EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrdm_dev_opt",
+           "qrdm_dgeqrf_dev",
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
            "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
            "qrdm_profile_read",
@@ -51,6 +60,10 @@ def lib():
         L.qrdm_dgeqrdm_dev_trace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
                                              ctypes.c_int, _ip, V, _ip, _dp, ctypes.c_int, V,
                                              ctypes.POINTER(Step), ctypes.c_int, _ip]
+        L.qrdm_dgeqrdm_dev_opt.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
+                                           ctypes.c_int, _ip, V, _ip, _dp, ctypes.c_int,
+                                           ctypes.POINTER(Opts), V, ctypes.POINTER(Step),
+                                           ctypes.c_int, _ip]
         L.qrdm_dgeqrf_dev.argtypes = [ctypes.c_int, ctypes.c_int, V, ctypes.c_int, V, V]
         L.qrdm_dormqr_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
                                       ctypes.c_int, V, V, ctypes.c_int, V]
@@ -73,7 +86,7 @@ def last_error():
 
 
 def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
-                layout=102):
+                layout=102, trailing="fp64"):
     """DM-QR in place of a device matrix.
 
     a: contiguous float64 device tensor of shape (n, lda) -- its memory is the
@@ -81,6 +94,8 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
     device tensor with min(m, n) entries; jpvt, ncols: int32 numpy arrays;
     thres: float64 numpy array [delta, tau(, eta)].  stream: hipStream_t as
     an int (torch.cuda.current_stream().cuda_stream), None = default stream.
+    trailing: precision of the trailing update's products, "fp64" (the
+    plain path), "fp32" or "bf16x3" (qrdm_dgeqrdm_dev_opt).
     Returns (info, steps), steps = per-step records when trace is set.
     """
     import numpy as np
@@ -95,11 +110,17 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
     args = (layout, m, n, ctypes.c_void_p(a.data_ptr()), lda, jpvt.ctypes.data_as(_ip),
             ctypes.c_void_p(tau.data_ptr()), ncols.ctypes.data_as(_ip),
             thres.ctypes.data_as(_dp), nb, s)
-    if trace:
-        cap = max(1, min(m, n))
-        tr = (Step * cap)()
+    if trailing not in TRAILING:
+        raise ValueError("trailing must be one of %s" % ", ".join(TRAILING))
+    if trace or trailing != "fp64":
+        cap = max(1, min(m, n)) if trace else 0
+        tr = (Step * max(cap, 1))()
         ns = ctypes.c_int(0)
-        info = L.qrdm_dgeqrdm_dev_trace(*args, tr, cap, ctypes.byref(ns))
+        opts = Opts(TRAILING[trailing], None)
+        info = L.qrdm_dgeqrdm_dev_opt(*args[:-1], ctypes.byref(opts), s,
+                                      tr if trace else None, cap, ctypes.byref(ns))
+        if not trace:
+            return info, None
         steps = [dict(j=t.j, kmax=t.kmax, nc=t.nc, fjb=t.fjb, fjb_cmp=t.fjb_cmp, nswap=t.nswap,
                       panel=t.panel, maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]))
                  for t in tr[:ns.value]]

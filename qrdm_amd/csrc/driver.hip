@@ -216,6 +216,7 @@ struct Args {
   qrdm_step* trace;
   int ntrace;
   int* nsteps;
+  int trail = TRAIL_FP64;  // precision of the trailing update's products
 };
 
 int check_args(const Args& x) {
@@ -362,13 +363,13 @@ int run_dm(const Args& x) {
     // follow in the step's end
     auto queue_update = [&](int jl) {
       int pl = g_prof.begin(PC_LARFB, s);
-      launch_gemm1(A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
+      launch_gemm1_prec(x.trail, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
       g_prof.end(pl, s, 0.0);
       int pw = g_prof.begin(PC_WSUM, s);
       launch_wsum(jl, m, jl + 1, n, B.T, 1, B.Wp, B.W, B.nsplit, s, B.st);
       g_prof.end(pw, s, 0.0);
       int pa = g_prof.begin(PC_APPLY, s);
-      launch_larfb_apply(A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.W, s, B.st);
+      launch_apply_prec(x.trail, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.W, s, B.st);
       g_prof.end(pa, s, 0.0);
       int pn = g_prof.begin(PC_NORM, s);
       launch_norm_update(A, lda, jl, m, n, B.work, B.workold, B.st, tol3z, s);
@@ -641,6 +642,24 @@ int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double* d_a, int lda
          (hipStream_t)stream, trace, ntrace, nsteps};
   int bad = check_args(x);
   if (bad) return bad;
+  return guarded(x);
+}
+
+int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double* d_a, int lda, int* jpvt,
+                         double* d_tau, int* ncols, const double* thres, int nb,
+                         const qrdm_opts* opts, void* stream, qrdm_step* trace, int ntrace,
+                         int* nsteps) {
+  Args x{matrix_layout, m, n, lda, nb, d_a, jpvt, d_tau, ncols, thres,
+         (hipStream_t)stream, trace, ntrace, nsteps};
+  int bad = check_args(x);
+  if (bad) return bad;
+  if (opts) {
+    if (opts->trailing < QRDM_TRAILING_FP64 || opts->trailing > QRDM_TRAILING_BF16X3) {
+      fprintf(stderr, "DGEQRDM: unknown trailing-update precision %d\n", opts->trailing);
+      return -1;
+    }
+    x.trail = opts->trailing;
+  }
   return guarded(x);
 }
 

@@ -180,6 +180,16 @@ void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const doubl
                     const DevState* st = nullptr);
 void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
                         const double* W, hipStream_t s, const DevState* st = nullptr);
+// Precision of the trailing update's two products (mixed.hip): FP64 runs
+// launch_gemm1 / launch_larfb_apply; FP32 and BF16X3 the matrix-core forms
+// with FP32 accumulation (C stays FP64).
+enum { TRAIL_FP64 = 0, TRAIL_FP32 = 1, TRAIL_BF16X3 = 2 };
+void launch_gemm1_prec(int prec, double* A, int lda, int j, int m, int c0, int n,
+                       const double* V, int ldv, double* Wp, int nsplit, hipStream_t s,
+                       const DevState* st);
+void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n,
+                       const double* V, int ldv, const double* W, hipStream_t s,
+                       const DevState* st);
 // launch_larfb_w's two launches: the partial products Wp = V^T C per row
 // slab, then W = op(T) sum Wp.
 void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
