@@ -103,6 +103,52 @@ def cpu_baseline(n_sample=8192, rank_sample=2048):
             "seconds": round(t_dm, 3), "dgeqp3_seconds": round(t_qp3, 3)}
 
 
+def batch_shapes(count, seed, mmin=64, mmax=1024, nmin=32, nmax=512):
+    """Uneven batch shapes: m uniform in [mmin, mmax], n uniform in
+    [nmin, min(m, nmax)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ms = rng.integers(mmin, mmax + 1, count)
+    return [(int(m), int(rng.integers(nmin, min(int(m), nmax) + 1))) for m in ms]
+
+
+def batch_fill(torch, plan, seed, lo=-10.0):
+    """The packed batch tensor of plan and the spectra: matrix i = U diag(s) V^T, s sorted
+    log-uniform over [10^lo, 1], U (m x k) and V (n x k) with orthonormal
+    columns from the QR of seeded Gaussians (grouped by row count so that
+    each group is one batched QR on the device)."""
+    import numpy as np
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed + 1)
+    shapes = plan.shapes
+    ks = [min(m, n) for m, n in shapes]
+    facs = [[None, None] for _ in shapes]
+    for side in (0, 1):
+        groups = {}
+        for i, (m, n) in enumerate(shapes):
+            groups.setdefault((m, n)[side], []).append(i)
+        for rows in sorted(groups):
+            idx = groups[rows]
+            kmax = max(ks[i] for i in idx)
+            g = torch.randn(len(idx), rows, kmax, generator=gen, device="cuda",
+                            dtype=torch.float64)
+            q = torch.linalg.qr(g).Q
+            for b, i in enumerate(idx):
+                facs[i][side] = q[b, :, :ks[i]]
+    buf = torch.empty(plan.a_size, dtype=torch.float64, device="cuda")
+    spectra = []
+    for i, (m, n) in enumerate(shapes):
+        k = ks[i]
+        s = np.sort(10.0 ** rng.uniform(lo, 0.0, k))[::-1].copy()
+        spectra.append(s)
+        st = torch.from_numpy(s).cuda()
+        a = (facs[i][0] * st) @ facs[i][1].T  # (m, n)
+        o = int(plan.a_off[i])
+        buf[o:o + m * n] = a.T.contiguous().view(-1)
+    return buf, spectra
+
+
 def backward_error(torch, qrdm_amd, work, pristine, tau, jp, stream):
     """||A P - Q R||_F / ||A||_F of the factorisation held in work (R and the
     reflectors, column-major m x n in an (n, m) tensor), Q R formed by the

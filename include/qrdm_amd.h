@@ -102,6 +102,51 @@ int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double *d_a, int lda, 
                          int *nsteps);
 
 /*
+ * Batched DM-QR: many independent factorisations (each with dgeqrdm's
+ * semantics, free columns, the column-by-column panel) in ONE persistent
+ * kernel launch.  No counterpart in the reference, which factorises one
+ * matrix per call.
+ *
+ * Matrix i is column-major m x n with leading dimension lda at
+ * d_a + a_off; its tau (min(m, n)) at d_tau + tau_off; its jpvt and ncols
+ * (n entries each, jpvt 1-based on exit) at d_jpvt + col_off and
+ * d_ncols + col_off; its info (0, or -13 for a non-finite entry, the matrix
+ * then untouched) at d_info[i].  All device pointers.  Limits: 1 <= n <= 512,
+ * m >= 1, lda >= m.
+ */
+typedef struct {
+  int m, n, lda, reserved;
+  long long a_off, tau_off, col_off;
+} qrdm_batch_item;
+
+typedef struct qrdm_batch_plan qrdm_batch_plan;
+
+/* Uploads the descriptors and builds the workgroup schedule (matrices in
+ * decreasing flop count, each to the least loaded workgroup).  nb: the DM
+ * block (k_max) of every factorisation, 1..64; nb <= 0 lets
+ * qrdm_batch_autotune choose it among 16, 32, 64 (it changes the pivots).
+ * Returns NULL on error (text through qrdm_last_error). */
+qrdm_batch_plan *qrdm_batch_plan_create(int count, const qrdm_batch_item *items, int nb);
+void qrdm_batch_plan_destroy(qrdm_batch_plan *plan);
+/* out[0..4) = {waves per workgroup, workgroups, nb, timed candidates}. */
+int qrdm_batch_plan_query(const qrdm_batch_plan *plan, int *out, int nout);
+/* Sets the launch shape by hand (waves per workgroup, 4 or 8). */
+int qrdm_batch_plan_set_shape(qrdm_batch_plan *plan, int waves);
+/* Times the launch shapes (4 or 8 waves per workgroup; and nb when the plan
+ * was created with nb <= 0) on a scratch copy of the batch d_a (left
+ * unchanged) and keeps the fastest.  thres = {delta, tau}.  Synchronous. */
+int qrdm_batch_autotune(qrdm_batch_plan *plan, const double *d_a, const double *thres,
+                        void *stream);
+/* Factorises the whole batch in place: one kernel launch on stream,
+ * asynchronous (results are valid once the stream has completed it).
+ * thres = {delta, tau, eta}; stop_mode as ncols[0] of dgeqrdm (0..3).
+ * use_graph != 0: the launch is captured once into a hipGraph (per set of
+ * pointers and parameters) and replayed with hipGraphLaunch. */
+int qrdm_dgeqrdm_batched(qrdm_batch_plan *plan, double *d_a, double *d_tau, int *d_jpvt,
+                         int *d_ncols, int *d_info, const double *thres, int stop_mode,
+                         int use_graph, void *stream);
+
+/*
  * Unpivoted Householder QR on the device (LAPACK dgeqrf semantics; the
  * reference's QRDM.QRF forwards to LAPACKE_dgeqrf, QRDM_wrapper.c:44-68).
  * d_a, d_tau device pointers.

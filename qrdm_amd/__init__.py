@@ -26,6 +26,13 @@ class Opts(ctypes.Structure):
 TRAILING = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 
 
+class BatchItem(ctypes.Structure):
+    """qrdm_batch_item of include/qrdm_amd.h."""
+    _fields_ = [("m", ctypes.c_int), ("n", ctypes.c_int), ("lda", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("a_off", ctypes.c_longlong),
+                ("tau_off", ctypes.c_longlong), ("col_off", ctypes.c_longlong)]
+
+
 class Step(ctypes.Structure):
     """qrdm_step of include/qrdm_amd.h."""
     _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
@@ -38,7 +45,8 @@ EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrd
            "qrdm_dgeqrf_dev",
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
            "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
-           "qrdm_profile_read",
+           "qrdm_profile_read", "qrdm_batch_plan_create", "qrdm_batch_plan_destroy",
+           "qrdm_batch_plan_query", "qrdm_batch_plan_set_shape", "qrdm_batch_autotune", "qrdm_dgeqrdm_batched",
            "qrdm_last_error", "qrdm_version")
 
 PROFILE_CLASSES = ("larfb_w", "norm_update", "select", "panel", "permute", "init_norms",
@@ -64,6 +72,14 @@ def lib():
                                            ctypes.c_int, _ip, V, _ip, _dp, ctypes.c_int,
                                            ctypes.POINTER(Opts), V, ctypes.POINTER(Step),
                                            ctypes.c_int, _ip]
+        L.qrdm_batch_plan_create.argtypes = [ctypes.c_int, ctypes.POINTER(BatchItem),
+                                             ctypes.c_int]
+        L.qrdm_batch_plan_create.restype = V
+        L.qrdm_batch_plan_destroy.argtypes = [V]
+        L.qrdm_batch_plan_query.argtypes = [V, _ip, ctypes.c_int]
+        L.qrdm_batch_plan_set_shape.argtypes = [V, ctypes.c_int]
+        L.qrdm_batch_autotune.argtypes = [V, V, _dp, V]
+        L.qrdm_dgeqrdm_batched.argtypes = [V, V, V, V, V, V, _dp, ctypes.c_int, ctypes.c_int, V]
         L.qrdm_dgeqrf_dev.argtypes = [ctypes.c_int, ctypes.c_int, V, ctypes.c_int, V, V]
         L.qrdm_dormqr_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
                                       ctypes.c_int, V, V, ctypes.c_int, V]
@@ -126,6 +142,85 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
                  for t in tr[:ns.value]]
         return info, steps
     return L.qrdm_dgeqrdm_dev(*args), None
+
+
+class BatchPlan:
+    """A batch of independent DM-QR factorisations run by one persistent
+    kernel launch (qrdm_batch_* of include/qrdm_amd.h).
+
+    shapes: list of (m, n); the matrices are packed one after another,
+    column-major with lda = m (a_off[i]), tau at tau_off[i], jpvt and ncols
+    at col_off[i].  nb: DM block 1..64, or <= 0 to let autotune() pick it.
+    """
+
+    def __init__(self, shapes, nb=64):
+        import numpy as np
+        self.shapes = [(int(m), int(n)) for m, n in shapes]
+        cnt = len(self.shapes)
+        self.items = (BatchItem * cnt)()
+        self.a_off = np.zeros(cnt, np.int64)
+        self.tau_off = np.zeros(cnt, np.int64)
+        self.col_off = np.zeros(cnt, np.int64)
+        a = t = c = 0
+        for i, (m, n) in enumerate(self.shapes):
+            self.items[i] = BatchItem(m, n, m, 0, a, t, c)
+            self.a_off[i], self.tau_off[i], self.col_off[i] = a, t, c
+            a += m * n
+            t += min(m, n)
+            c += n
+        self.a_size, self.tau_size, self.col_size = a, t, c
+        self._p = lib().qrdm_batch_plan_create(cnt, self.items, nb)
+        if not self._p:
+            raise ValueError(last_error())
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().qrdm_batch_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def query(self):
+        import numpy as np
+        out = np.zeros(4, np.int32)
+        lib().qrdm_batch_plan_query(self._p, out.ctypes.data_as(_ip), 4)
+        return dict(waves=int(out[0]), grid=int(out[1]), nb=int(out[2]), tuned=int(out[3]))
+
+    def set_shape(self, waves):
+        """Launch shape by hand: 4 or 8 waves per workgroup."""
+        if lib().qrdm_batch_plan_set_shape(self._p, waves) != 0:
+            raise ValueError("waves must be 4 or 8")
+
+    def autotune(self, a, thres=(0.9, 0.15), stream=None):
+        """Times the launch shapes on a scratch copy of the batch tensor a."""
+        import numpy as np
+        th = np.ascontiguousarray(list(thres) + [0.0] * (3 - len(thres)), dtype=np.float64)
+        info = lib().qrdm_batch_autotune(self._p, ctypes.c_void_p(a.data_ptr()),
+                                         th.ctypes.data_as(_dp),
+                                         ctypes.c_void_p(stream if stream else 0))
+        if info != 0:
+            raise RuntimeError("qrdm_batch_autotune: %d %s" % (info, last_error()))
+        return self.query()
+
+    def run(self, a, tau, jpvt, ncols, info, thres=(0.9, 0.15, 0.0), stop=0, graph=True,
+            stream=None):
+        """Factorises the batch in place (float64 a, tau; int32 jpvt, ncols,
+        info device tensors); asynchronous on stream."""
+        import numpy as np
+        th = np.ascontiguousarray(list(thres) + [0.0] * (3 - len(thres)), dtype=np.float64)
+        r = lib().qrdm_dgeqrdm_batched(self._p, ctypes.c_void_p(a.data_ptr()),
+                                       ctypes.c_void_p(tau.data_ptr()),
+                                       ctypes.c_void_p(jpvt.data_ptr()),
+                                       ctypes.c_void_p(ncols.data_ptr()),
+                                       ctypes.c_void_p(info.data_ptr()), th.ctypes.data_as(_dp),
+                                       stop, 1 if graph else 0,
+                                       ctypes.c_void_p(stream if stream else 0))
+        if r != 0:
+            raise RuntimeError("qrdm_dgeqrdm_batched: %d %s" % (r, last_error()))
 
 
 def profile(enable=None, reset=False, classes=None):

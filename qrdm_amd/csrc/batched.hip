@@ -1,0 +1,430 @@
+// batched.hip -- many independent DM-QR factorisations in one persistent
+// launch: each workgroup factorises whole matrices, one after another, from
+// a precomputed schedule (host longest-processing-time order over the
+// matrices' flop counts), with the DM state of the current matrix in LDS.
+//
+// The arithmetic of a step is the plain path's (src/dgeqrdm_work.c:694-787)
+// with the column-by-column panel: the same device routines for the norms
+// (wave_nrm2), the top-k order (wave_sort64 / wave_merge64), the greedy
+// acceptance and permute_marked plan (select_wave), dgeqr2_mia
+// (panel_exact_body) and dlarft (larft_recurrence); the candidate Gram, V^T V
+// and the trailing update run on the FP64 matrix cores inside the workgroup.
+// Limits: n <= BT_NMAX (the per-column state lives in LDS); m is free.
+
+namespace qrdm {
+
+constexpr int BT_NMAX = BATCH_NMAX;
+constexpr double kEpsD = 1.1102230246251565e-16;  // LAPACK dlamch('E') = 2^-53
+
+// One matrix of a batch; layout of qrdm_batch_item (include/qrdm_amd.h).
+struct BatchItem {
+  int m, n, lda, pad;
+  long long a_off, tau_off, col_off;
+};
+
+struct BatchParams {
+  double delta, tau_dm, eta3, tol3z;
+  int nb, stop_mode;
+};
+
+template <int NW>
+struct BatchLds {
+  double work[BT_NMAX], wold[BT_NMAX];
+  int jp[BT_NMAX];
+  double lv[NW][KMAX];
+  int li[NW][KMAX];
+  double R1[KMAX * (KMAX + 1)];  // Gram slab / G / V^T V / trailing-update W
+  double Ts[KMAX * KMAX];
+  double ts[KMAX], tmp[KMAX];
+  double red[16];
+  int perm[4 * KMAX + KMAX];
+  DevState st;
+  int s_int[4];
+};
+
+// Unit lower trapezoidal V of the panel at column j: V(row, c) for c < kk.
+__device__ __forceinline__ double bt_v(const double* a, int lda, int j, int kk, int row, int c,
+                                       int m) {
+  const int rr = row - j;
+  if (c >= kk || row >= m || rr < c) return 0.0;
+  return rr == c ? 1.0 : a[(size_t)(j + c) * lda + row];
+}
+
+// C := C - V T^T V^T C on rows [j, m), columns [c0, n), in 64-column tiles.
+// P = V^T C: wave w takes the 16-row chunks w % 4, w % 4 + 4, ... and the
+// 16-column groups u of its share (all four with 4 waves; two, by w / 4, with
+// 8), accumulating its partial on the matrix cores (lane group g holds rows
+// 4g..4g+3 of a chunk, as k_gemm1); the partials are summed in wave order in
+// LDS.  W = T^T P (waves 0..3, rows 16w..), then C -= V W as C^T -= W^T V^T,
+// wave w taking the 16-row blocks w, w + NW, ...
+template <int NW>
+__device__ void bt_trailing(double* a, int lda, int j, int m, int kk, int c0, int n,
+                            BatchLds<NW>& L) {
+  constexpr int UPW = 16 / NW;  // 16-column groups per wave in P
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, li = lane & 15;
+  const int wr = wave & 3, u0 = UPW * (wave >> 2);
+  double* Wl = L.R1;  // [k][c], leading dimension KMAX + 1
+  // V columns of this lane (16t + li), clamped to a valid column
+  const double* vcol[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) vcol[t] = a + (size_t)(j + min(16 * t + li, kk - 1)) * lda;
+  for (int cb = c0; cb < n; cb += 64) {
+    const int ncl = min(64, n - cb);
+    const double* ccol[UPW];
+#pragma unroll
+    for (int u = 0; u < UPW; u++) ccol[u] = a + (size_t)(cb + min(16 * (u0 + u) + li, ncl - 1)) * lda;
+    dbl4 acc[4][UPW];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int u = 0; u < UPW; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    for (int rs = j + 16 * wr; rs < m; rs += 64) {
+      double vr[4][4], cr[UPW][4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = rs + 4 * g + q, rr = row - j;
+        const bool rok = row < m;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int c = 16 * t + li;
+          const double x = (rok && c < kk && rr > c) ? vcol[t][row] : 0.0;
+          vr[t][q] = (rr == c && c < kk) ? 1.0 : x;
+        }
+#pragma unroll
+        for (int u = 0; u < UPW; u++)
+          cr[u][q] = (rok && 16 * (u0 + u) + li < ncl) ? ccol[u][row] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int u = 0; u < UPW; u++)
+            acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vr[t][q], cr[u][q], acc[t][u], 0, 0,
+                                                             0);
+    }
+    // sum of the wave partials, in wave order: P(16t + g + 4v, 16u + li)
+    __syncthreads();  // the previous tile is done with Wl
+    for (int w = 0; w < NW; w++) {
+      if (wave == w) {
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+#pragma unroll
+          for (int u = 0; u < UPW; u++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+              double* p = &Wl[(16 * t + g + 4 * v) * (KMAX + 1) + 16 * (u0 + u) + li];
+              *p = (wr == 0) ? acc[t][u][v] : *p + acc[t][u][v];
+            }
+      }
+      __syncthreads();
+    }
+    // W = T^T P: T(i, r) = Ts[i + r * KMAX]
+    dbl4 wacc[4];
+    if (wave < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) wacc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+      for (int k0 = 0; k0 < KMAX; k0 += 4) {
+        const double av = L.Ts[(k0 + g) + (16 * wave + li) * KMAX];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          wacc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+              av, Wl[(k0 + g) * (KMAX + 1) + 16 * u + li], wacc[u], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    if (wave < 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) Wl[(16 * wave + g + 4 * v) * (KMAX + 1) + 16 * u + li] = wacc[u][v];
+    }
+    __syncthreads();
+    // C -= V W on 16-row blocks: D_u(c, r) = sum_k W(k, 16u + c) V(r, k)
+    const double* cbase = a + (size_t)cb * lda;
+    for (int rb = j + 16 * wave; rb < m; rb += 16 * NW) {
+      const int row = rb + li;
+      double vk[KMAX / 4];
+#pragma unroll
+      for (int s = 0; s < KMAX / 4; s++) vk[s] = bt_v(a, lda, j, kk, row, 4 * s + g, m);
+      double cval[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = 16 * u + g + 4 * v;
+          cval[u][v] = (c < ncl && row < m) ? cbase[(size_t)c * lda + row] : 0.0;
+        }
+      dbl4 d[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) d[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < KMAX / 4; s++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          d[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(Wl[(4 * s + g) * (KMAX + 1) + 16 * u + li],
+                                                     vk[s], d[u], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int c = 16 * u + g + 4 * v;
+          if (c < ncl && row < m) a[(size_t)(cb + c) * lda + row] = cval[u][v] - d[u][v];
+        }
+    }
+  }
+}
+
+// Gram X^T X of up to 64 columns over rows [r0, m), X(r, c) = colval(c, r),
+// into R1 (leading dimension KMAX + 1); waves 0..3 run the matrix cores,
+// every wave loads.
+template <int NW, typename ColFn>
+__device__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L) {
+  constexpr int NT = 64 * NW;
+  double* X = L.R1;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  dbl4 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  for (int rs = r0; rs < m; rs += GRAM_ROWS) {
+    double xv[KMAX * GRAM_ROWS / NT];
+#pragma unroll
+    for (int q = 0; q < KMAX * GRAM_ROWS / NT; q++) {
+      const int e = tid + NT * q, c = e / GRAM_ROWS, r = rs + e % GRAM_ROWS;
+      xv[q] = (c < ncols && r < m) ? colval(c, r) : 0.0;
+    }
+    __syncthreads();  // the previous slab's products are done with X
+#pragma unroll
+    for (int q = 0; q < KMAX * GRAM_ROWS / NT; q++) {
+      const int e = tid + NT * q;
+      X[(e / GRAM_ROWS) * (GRAM_ROWS + 1) + e % GRAM_ROWS] = xv[q];
+    }
+    __syncthreads();
+    if (wave < 4) gram_slab_acc(X, acc);
+  }
+  __syncthreads();
+  if (wave < 4) {
+    const int lane = tid & 63, g = lane >> 4, li = lane & 15;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        X[(16 * wave + g + 4 * v) + (16 * u + li) * (KMAX + 1)] = acc[u][v];
+  }
+  __syncthreads();
+}
+
+// One DM-QR (dgeqrdm semantics, free columns) of the matrix it by the whole
+// workgroup.  Returns info (0, or -13 for a non-finite entry, A untouched).
+template <int NW>
+__device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* jp_all,
+                         int* nc_all, const BatchParams& P, BatchLds<NW>& L) {
+  constexpr int NT = 64 * NW;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = it.m, n = it.n, lda = it.lda;
+  double* a = A + it.a_off;
+  double* tau = tau_all + it.tau_off;
+  const int minmn = m < n ? m : n;
+  // initial norms (src/dgeqrdm_work.c:670-682), as k_init_norms
+  double mx = 0.0, bad = 0.0;
+  for (int c = wave; c < n; c += NW) {
+    const double nr = wave_nrm2(a + (size_t)c * lda, m);
+    if (lane == 0) {
+      L.work[c] = nr;
+      L.wold[c] = nr;
+    }
+    if (!(nr <= DBL_MAX)) bad = 1.0;
+    else mx = fmax(mx, nr);
+  }
+  for (int c = tid; c < n; c += NT) L.jp[c] = c + 1;
+  const double maxnrm0 = block_max(mx, L.red);
+  if (block_max(bad, L.red) != 0.0) return -13;
+  double eta = 0.0;
+  const int enable_stop = P.stop_mode;
+  if (P.stop_mode == 1) eta = kEpsD * n;
+  else if (P.stop_mode == 2) eta = kEpsD * sqrt((double)n);
+  else if (P.stop_mode == 3) eta = P.eta3;
+  eta *= maxnrm0;
+  int j = 0, itn = 0;
+  while (j < minmn) {
+    const int nsub = n - j;
+    int kmax = P.nb < KMAX ? P.nb : KMAX;
+    if (m - j < kmax) kmax = m - j;
+    if (nsub < kmax) kmax = nsub;
+    // --- candidates: the first kmax keys of work[j..n) in qsort/cmpStruct
+    // order (value descending, position ascending), count > tau * max
+    {
+      double v = -1.0;
+      int ix = 0x7fffffc0 + lane;  // a sorted list of padding keys
+      for (int b = wave; 64 * b < nsub; b += NW) {
+        const int t = 64 * b + lane;
+        double bv = (t < nsub) ? L.work[j + t] : -1.0;
+        int bi = (t < nsub) ? t : 0x40000000 + t;
+        wave_sort64(bv, bi);
+        wave_merge64(v, ix, bv, bi);
+      }
+      L.lv[wave][lane] = v;
+      L.li[wave][lane] = ix;
+      __syncthreads();
+      block_merge_lists<NW>(L.lv, L.li, v, ix);
+      if (wave == 0) {
+        const double cv = L.lv[0][lane];
+        const int ci = L.li[0][lane];
+        L.st.cand_val[lane] = lane < kmax ? cv : 0.0;
+        L.st.cand_idx[lane] = lane < kmax ? ci : -1;
+        const double lim = P.tau_dm * L.lv[0][0];
+        const int nc = __popcll(__ballot(lane < kmax && cv > lim));
+        if (lane == 0) {
+          L.st.nc = nc;
+          L.st.kmax = kmax;
+          L.st.j = j;
+          L.st.halt = 0;
+        }
+      }
+      __syncthreads();
+    }
+    const int nc = L.st.nc;
+    // --- cosines (src/dgeqrdm_work.c:365-379): Gram of the candidate
+    // columns over rows j..m, scaled by the reciprocal norms in select_wave
+    if (nc > 1) {
+      const DevState& st = L.st;
+      bt_gram<NW>(j, m, nc,
+                  [&](int c, int r) { return a[(size_t)(j + st.cand_idx[c]) * lda + r]; }, L);
+    }
+    // --- greedy acceptance and the permute_marked plan
+    if (wave == 0) select_wave(L.R1, P.delta, m, n, INT_MAX, 1, L.perm, &L.st);
+    __syncthreads();
+    const int fjb = L.st.fjb, npair = L.st.npair;
+    // --- column moves (whole columns; work and jpvt with them, not workold)
+    if (npair > 0) {
+      const int rb_rows = max(1, min(m, (8 * NT) / npair));
+      for (int r0 = 0; r0 < m; r0 += rb_rows) {
+        const int rows = min(rb_rows, m - r0), tot = npair * rows;
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int e = tid + NT * q;
+          v[q] = (e < tot) ? a[(size_t)L.perm[2 * (e / rows) + 1] * lda + r0 + e % rows] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const int e = tid + NT * q;
+          if (e < tot) a[(size_t)L.perm[2 * (e / rows)] * lda + r0 + e % rows] = v[q];
+        }
+      }
+      double wv = 0.0;
+      int jv = 0;
+      if (tid < npair) {
+        wv = L.work[L.perm[2 * tid + 1]];
+        jv = L.jp[L.perm[2 * tid + 1]];
+      }
+      __syncthreads();
+      if (tid < npair) {
+        L.work[L.perm[2 * tid]] = wv;
+        L.jp[L.perm[2 * tid]] = jv;
+      }
+      __syncthreads();
+    }
+    // --- dgeqr2_mia with truncation (src/dgeqrdm_work.c:735-738)
+    const int kk = panel_exact_body(a, lda, j, m, fjb, P.tau_dm, 1, tau + j, L.red);
+    __syncthreads();
+    // --- dlarft from S = V^T V (src/dgeqrdm_work.c:751-754)
+    const int c0 = j + fjb;
+    if (c0 < n) {
+      bt_gram<NW>(j, m, kk, [&](int c, int r) { return bt_v(a, lda, j, kk, r, c, m); }, L);
+      // R1 holds S with leading dimension KMAX + 1: repack to KMAX for the recurrence
+      double sv[KMAX * KMAX / NT];
+#pragma unroll
+      for (int q = 0; q < KMAX * KMAX / NT; q++) {
+        const int e = tid + NT * q;
+        sv[q] = L.R1[(e & 63) + (e >> 6) * (KMAX + 1)];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < KMAX * KMAX / NT; q++) {
+        const int e = tid + NT * q;
+        L.R1[e] = sv[q];
+        L.Ts[e] = 0.0;
+      }
+      if (tid < KMAX) L.ts[tid] = tid < kk ? tau[j + tid] : 0.0;
+      __syncthreads();
+      larft_recurrence(L.R1, L.Ts, L.ts, L.tmp, kk);
+      __syncthreads();
+      // --- dlarfb on columns j+fjb..n (src/dgeqrdm_work.c:762-767)
+      bt_trailing<NW>(a, lda, j, m, kk, c0, n, L);
+      __syncthreads();
+    }
+    // --- norm_update (src/dgeqrdm_work.c:80-109), as k_norm_update
+    double nmx = 0.0;
+    const int mrem = m - j - kk;
+    for (int col = j + kk + wave; col < n; col += NW) {
+      double wk = L.work[col];
+      if (wk == 0.0) continue;
+      const double* ac = a + (size_t)col * lda + j;
+      const double x = (lane < kk) ? ac[lane] : 0.0;
+      const double s = wave_sum(x * x);
+      double temp = sqrt(fabs(s)) / wk;
+      const double d2 = (temp + 1.0) * (1.0 - temp);
+      temp = (0.0 >= d2) ? 0.0 : d2;
+      const double d1 = wk / L.wold[col];
+      const double temp2 = temp * (d1 * d1);
+      if (temp2 <= P.tol3z) {
+        wk = (mrem > 0) ? wave_nrm2(ac + kk, mrem) : 0.0;
+        if (lane == 0) L.wold[col] = wk;
+      } else {
+        wk *= sqrt(temp);
+      }
+      if (lane == 0) L.work[col] = wk;
+      if (wk == wk) nmx = fmax(nmx, wk);
+    }
+    const double maxnrm = block_max(nmx, L.red);
+    // --- step end (src/dgeqrdm_work.c:740, :780-785)
+    if (tid == 0) nc_all[it.col_off + itn] = kk;
+    itn++;
+    int done = (j + kk >= minmn) ? 1 : 0;
+    if (enable_stop && maxnrm * sqrt((double)(n - j - kk)) <= eta) done = 1;
+    j += kk;
+    __syncthreads();
+    if (done) break;
+  }
+  for (int c = tid; c < n; c += NT) jp_all[it.col_off + c] = L.jp[c];
+  return 0;
+}
+
+// Persistent batch kernel: workgroup w factorises the matrices
+// sched[sched_off[w] .. sched_off[w + 1]) in order.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_dm_batch(const BatchItem* __restrict__ items,
+                                                      const int* __restrict__ sched,
+                                                      const int* __restrict__ sched_off,
+                                                      double* A, double* tau, int* jpvt,
+                                                      int* ncols, int* info, BatchParams P) {
+  __shared__ BatchLds<NW> L;
+  const int b0 = sched_off[blockIdx.x], b1 = sched_off[blockIdx.x + 1];
+  for (int b = b0; b < b1; b++) {
+    const int idx = sched[b];
+    const BatchItem it = items[idx];
+    const int r = bt_factor<NW>(it, A, tau, jpvt, ncols, P, L);
+    if (threadIdx.x == 0) info[idx] = r;
+    __syncthreads();
+  }
+}
+
+void launch_dm_batch(int waves, int grid, const void* items, const int* sched,
+                     const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
+                     int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
+                     double tol3z, hipStream_t s) {
+  BatchParams P{delta, tau_dm, eta3, tol3z, nb, stop_mode};
+  const BatchItem* it = static_cast<const BatchItem*>(items);
+  if (waves == 8)
+    hipLaunchKernelGGL(k_dm_batch<8>, dim3(grid), dim3(512), 0, s, it, sched, sched_off, A, tau,
+                       jpvt, ncols, info, P);
+  else
+    hipLaunchKernelGGL(k_dm_batch<4>, dim3(grid), dim3(256), 0, s, it, sched, sched_off, A, tau,
+                       jpvt, ncols, info, P);
+}
+
+}  // namespace qrdm

@@ -587,6 +587,96 @@ void to_host(double* h, int ldh, const DevBlock& b, int rows, int cols) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Batched DM-QR: a plan holds the matrices' descriptors on the device, the
+// per-workgroup schedule and the launch shape; a run is one launch of
+// k_dm_batch (optionally replayed from a captured hipGraph).
+struct qrdm_batch_plan {
+  int count = 0;
+  int nb_req = 0;   // the caller's nb (<= 0: picked by qrdm_batch_autotune)
+  int nb_run = 64;  // nb of the runs
+  int waves = 4;    // 64-thread waves per workgroup (4 or 8)
+  int grid = 0;
+  std::vector<qrdm_batch_item> items;
+  void* d_items = nullptr;
+  int* d_sched = nullptr;
+  int* d_off = nullptr;
+  size_t a_ext = 0, tau_ext = 0, col_ext = 0;  // elements the batch spans
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct Key {
+    const void *a, *tau, *jp, *nc, *info;
+    double th[3];
+    int stop, waves, nb;
+    bool operator==(const Key& o) const {
+      return a == o.a && tau == o.tau && jp == o.jp && nc == o.nc && info == o.info &&
+             th[0] == o.th[0] && th[1] == o.th[1] && th[2] == o.th[2] && stop == o.stop &&
+             waves == o.waves && nb == o.nb;
+    }
+  } key{};
+  double tuned_ms[8] = {};
+  int tuned_n = 0;
+};
+
+namespace {
+
+// Flop count of one factorisation (2 m n^2 - 2 n^3 / 3 for m >= n).
+double qr_flops(int m, int n) {
+  const double a = m > n ? m : n, b = m > n ? n : m;
+  return 2.0 * a * b * b - 2.0 * b * b * b / 3.0;
+}
+
+// Longest-processing-time schedule: matrices in decreasing cost, each to the
+// least loaded workgroup (ties: lowest index); each workgroup's list keeps
+// that order.
+void batch_schedule(const std::vector<qrdm_batch_item>& it, int grid, std::vector<int>& sched,
+                    std::vector<int>& off) {
+  const int cnt = (int)it.size();
+  std::vector<int> order(cnt);
+  for (int i = 0; i < cnt; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return qr_flops(it[x].m, it[x].n) > qr_flops(it[y].m, it[y].n);
+  });
+  std::vector<std::vector<int>> lists(grid);
+  std::vector<std::pair<double, int>> heap;  // (load, wg), min-heap
+  for (int w = 0; w < grid; w++) heap.emplace_back(0.0, w);
+  auto cmp = [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+    return x.first > y.first || (x.first == y.first && x.second > y.second);
+  };
+  std::make_heap(heap.begin(), heap.end(), cmp);
+  for (int i : order) {
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    auto& top = heap.back();
+    lists[top.second].push_back(i);
+    top.first += qr_flops(it[i].m, it[i].n);
+    std::push_heap(heap.begin(), heap.end(), cmp);
+  }
+  sched.clear();
+  off.assign(grid + 1, 0);
+  for (int w = 0; w < grid; w++) {
+    off[w] = (int)sched.size();
+    sched.insert(sched.end(), lists[w].begin(), lists[w].end());
+  }
+  off[grid] = (int)sched.size();
+}
+
+void batch_release_graph(qrdm_batch_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->exec = nullptr;
+  p->graph = nullptr;
+}
+
+void batch_launch(qrdm_batch_plan* p, int waves, int nb, double* a, double* tau, int* jp,
+                  int* nc, int* info, const double* th, int stop, hipStream_t s) {
+  launch_dm_batch(waves, p->grid, p->d_items, p->d_sched, p->d_off, a, tau, jp, nc, info, nb,
+                  th[0], th[1], stop, th[2], sqrt(kEps), s);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* qrdm_version(void) { return "qrdm_amd 0.1 (gfx950)"; }
@@ -762,4 +852,174 @@ int dgeqrdm(int matrix_layout, int m, int n, double* a, int lda, int* jpvt, doub
   return info;
 }
 
+
+qrdm_batch_plan* qrdm_batch_plan_create(int count, const qrdm_batch_item* items, int nb) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err.clear();
+  if (count <= 0 || !items) {
+    g_err = "batch: count must be positive";
+    return nullptr;
+  }
+  if (nb > KMAX) {
+    g_err = "batch: nb exceeds 64";
+    return nullptr;
+  }
+  auto* p = new qrdm_batch_plan;
+  p->count = count;
+  p->nb_req = nb;
+  p->nb_run = nb > 0 ? nb : KMAX;
+  p->items.assign(items, items + count);
+  for (int i = 0; i < count; i++) {
+    const qrdm_batch_item& t = items[i];
+    if (t.m < 1 || t.n < 1 || t.n > BATCH_NMAX || t.lda < t.m || t.a_off < 0 || t.tau_off < 0 ||
+        t.col_off < 0) {
+      g_err = "batch: item " + std::to_string(i) +
+              " needs m >= 1, 1 <= n <= 512, lda >= m and non-negative offsets";
+      delete p;
+      return nullptr;
+    }
+    const int k = t.m < t.n ? t.m : t.n;
+    p->a_ext = std::max(p->a_ext, (size_t)t.a_off + (size_t)t.lda * (t.n - 1) + t.m);
+    p->tau_ext = std::max(p->tau_ext, (size_t)t.tau_off + k);
+    p->col_ext = std::max(p->col_ext, (size_t)t.col_off + t.n);
+  }
+  try {
+    int dev = 0, cus = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    p->grid = std::max(1, std::min(cus, count));
+    std::vector<int> sched, off;
+    batch_schedule(p->items, p->grid, sched, off);
+    HIPCHK(hipMalloc(&p->d_items, sizeof(qrdm_batch_item) * count));
+    HIPCHK(hipMalloc(&p->d_sched, sizeof(int) * sched.size()));
+    HIPCHK(hipMalloc(&p->d_off, sizeof(int) * off.size()));
+    HIPCHK(hipMemcpy(p->d_items, items, sizeof(qrdm_batch_item) * count, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->d_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+  } catch (const HipError& e) {
+    g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+    qrdm_batch_plan_destroy(p);
+    return nullptr;
+  }
+  return p;
+}
+
+void qrdm_batch_plan_destroy(qrdm_batch_plan* p) {
+  if (!p) return;
+  batch_release_graph(p);
+  if (p->cap) (void)hipStreamDestroy(p->cap);
+  if (p->d_items) (void)hipFree(p->d_items);
+  if (p->d_sched) (void)hipFree(p->d_sched);
+  if (p->d_off) (void)hipFree(p->d_off);
+  delete p;
+}
+
+int qrdm_batch_plan_query(const qrdm_batch_plan* p, int* out, int nout) {
+  if (!p || !out) return -1;
+  const double vals[4] = {(double)p->waves, (double)p->grid, (double)p->nb_run,
+                          (double)p->tuned_n};
+  for (int i = 0; i < nout && i < 4; i++) out[i] = (int)vals[i];
+  return 4;
+}
+
+int qrdm_batch_plan_set_shape(qrdm_batch_plan* p, int waves) {
+  if (!p || (waves != 4 && waves != 8)) return -1;
+  p->waves = waves;
+  batch_release_graph(p);
+  return 0;
+}
+
+int qrdm_batch_autotune(qrdm_batch_plan* p, const double* d_a, const double* thres,
+                        void* stream) {
+  if (!p || !d_a || !thres) return -1;
+  return guarded_call([&] {
+    hipStream_t s = (hipStream_t)stream;
+    double *a = nullptr, *tau = nullptr;
+    int *jp = nullptr, *nc = nullptr, *info = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (a) (void)hipFree(a);
+      if (tau) (void)hipFree(tau);
+      if (jp) (void)hipFree(jp);
+      if (nc) (void)hipFree(nc);
+      if (info) (void)hipFree(info);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+      HIPCHK(hipMalloc(&a, sizeof(double) * p->a_ext));
+      HIPCHK(hipMalloc(&tau, sizeof(double) * p->tau_ext));
+      HIPCHK(hipMalloc(&jp, sizeof(int) * p->col_ext));
+      HIPCHK(hipMalloc(&nc, sizeof(int) * p->col_ext));
+      HIPCHK(hipMalloc(&info, sizeof(int) * p->count));
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      const int nbs_auto[3] = {16, 32, 64};
+      const int nnb = p->nb_req > 0 ? 1 : 3;
+      float best = 1e30f;
+      int bw = p->waves, bnb = p->nb_run;
+      p->tuned_n = 0;
+      for (int ib = 0; ib < nnb; ib++) {
+        const int nb = p->nb_req > 0 ? p->nb_req : nbs_auto[ib];
+        for (int waves : {4, 8}) {
+          float t = 0.f;
+          for (int rep = 0; rep < 2; rep++) {  // the first run warms up
+            HIPCHK(hipMemcpyAsync(a, d_a, sizeof(double) * p->a_ext, hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipEventRecord(e0, s));
+            batch_launch(p, waves, nb, a, tau, jp, nc, info, thres, 0, s);
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+          }
+          if (p->tuned_n < 8) p->tuned_ms[p->tuned_n++] = t;
+          if (t < best) {
+            best = t;
+            bw = waves;
+            bnb = nb;
+          }
+        }
+      }
+      p->waves = bw;
+      p->nb_run = bnb;
+      batch_release_graph(p);
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+    return 0;
+  });
+}
+
+int qrdm_dgeqrdm_batched(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_jpvt,
+                         int* d_ncols, int* d_info, const double* thres, int stop_mode,
+                         int use_graph, void* stream) {
+  if (!p || !d_a || !d_tau || !d_jpvt || !d_ncols || !d_info || !thres) return -1;
+  if (thres[0] < 0.0 || thres[0] > 1.0 || thres[1] < 0.0 || thres[1] > 1.0 || stop_mode < 0 ||
+      stop_mode > 3)
+    return -1;
+  return guarded_call([&] {
+    hipStream_t s = (hipStream_t)stream;
+    const double th[3] = {thres[0], thres[1], stop_mode == 3 ? thres[2] : 0.0};
+    if (!use_graph) {
+      batch_launch(p, p->waves, p->nb_run, d_a, d_tau, d_jpvt, d_ncols, d_info, th, stop_mode, s);
+      return 0;
+    }
+    qrdm_batch_plan::Key k{d_a, d_tau, d_jpvt, d_ncols, d_info, {th[0], th[1], th[2]},
+                           stop_mode, p->waves, p->nb_run};
+    if (!p->exec || !(k == p->key)) {
+      batch_release_graph(p);
+      HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+      launch_dm_batch(p->waves, p->grid, p->d_items, p->d_sched, p->d_off, d_a, d_tau, d_jpvt,
+                      d_ncols, d_info, p->nb_run, th[0], th[1], stop_mode, th[2], sqrt(kEps),
+                      p->cap);
+      HIPCHK(hipStreamEndCapture(p->cap, &p->graph));
+      HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+      p->key = k;
+    }
+    HIPCHK(hipGraphLaunch(p->exec, s));
+    return 0;
+  });
+}
 }  // extern "C"

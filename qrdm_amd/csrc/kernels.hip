@@ -854,16 +854,10 @@ void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const i
 // thres = 5e-14, or tau_dm*|R(0,0)| once column 0 is done (tau_dm > 0).
 // truncate = 0 disables the test (unpivoted QR).  RUN_EXACT: j and fjb from
 // st, tau indexed from column 0 (the step writes tau[j..]).
-__global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j, int m,
-                                                      int fjb, double tau_dm, int truncate,
-                                                      double* tau, DevState* st, int mode) {
-  __shared__ double red[16];
-  if (!step_runs(st, mode)) return;
-  if (mode != RUN_ALWAYS) {
-    j = st->j;
-    fjb = st->fjb;
-    tau += j;
-  }
+// The panel's work for the whole workgroup (any size up to 1024 threads;
+// red holds blockDim.x / 64 doubles): returns fjb_cmp, the columns reduced.
+__device__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, double tau_dm,
+                                int truncate, double* tau, double* red) {
   const int msub = m - j;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
   double* P = A + j + (size_t)j * lda;
@@ -919,7 +913,21 @@ __global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j,
       __syncthreads();
     }
   }
-  if (tid == 0) {
+  return i;
+}
+
+__global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j, int m,
+                                                      int fjb, double tau_dm, int truncate,
+                                                      double* tau, DevState* st, int mode) {
+  __shared__ double red[16];
+  if (!step_runs(st, mode)) return;
+  if (mode != RUN_ALWAYS) {
+    j = st->j;
+    fjb = st->fjb;
+    tau += j;
+  }
+  const int i = panel_exact_body(A, lda, j, m, fjb, tau_dm, truncate, tau, red);
+  if (threadIdx.x == 0) {
     st->fjb_cmp = i;
     // dlarf_ inside the panel already updated the rejected columns
     // j+fjb_cmp..j+fjb-1; the trailing update starts at j+fjb (:764-766)
@@ -1022,6 +1030,28 @@ void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState*
 // T(0:i,i) = -tau_i T(0:i,0:i) (V^T v_i)(0:i), T(i,i) = tau_i, from S = V^T V.
 // With nslab > 0, S is first summed from nslab partial Grams (the exact
 // panel's V slabs); 256 threads, the recurrence on the first 64.
+// The recurrence for the whole workgroup (at least 64 threads): Ss = V^T V
+// and Ts (zero on entry) column-major with leading dimension KMAX, ts the
+// kk scalars tau, tmp KMAX doubles; all in LDS.
+__device__ void larft_recurrence(const double* Ss, double* Ts, const double* ts, double* tmp,
+                                 int kk) {
+  const int p = threadIdx.x;
+  for (int i = 0; i < kk; i++) {
+    const double ti = ts[i];
+    if (ti != 0.0) {
+      if (p < KMAX) tmp[p] = (p < i) ? -ti * Ss[p + i * KMAX] : 0.0;
+      __syncthreads();
+      double sacc = 0.0;
+      if (p < i)
+        for (int q = p; q < i; q++) sacc += Ts[p + q * KMAX] * tmp[q];
+      __syncthreads();
+      if (p < i) Ts[p + i * KMAX] = sacc;
+      if (p == i) Ts[i + i * KMAX] = ti;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(256) void k_larft(const double* __restrict__ S,
                                                const double* __restrict__ tau,
                                                const DevState* st, double* T, int mode,
@@ -1031,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_larft(const double* __restrict__ S,
   __shared__ double tmp[KMAX], ts[KMAX];
   if (!step_runs(st, mode)) return;
   if (mode != RUN_ALWAYS) tau += st->j;
-  const int tid = threadIdx.x, p = tid;
+  const int tid = threadIdx.x;
   const int kk = st->fjb_cmp;
   for (int e = tid; e < KMAX * KMAX; e += 256) {
     double v = 0.0;
@@ -1052,20 +1082,7 @@ __global__ __launch_bounds__(256) void k_larft(const double* __restrict__ S,
   }
   if (tid < KMAX) ts[tid] = tid < kk ? tau[tid] : 0.0;
   __syncthreads();
-  for (int i = 0; i < kk; i++) {
-    const double ti = ts[i];
-    if (ti != 0.0) {
-      if (p < KMAX) tmp[p] = (p < i) ? -ti * Ss[p + i * KMAX] : 0.0;
-      __syncthreads();
-      double sacc = 0.0;
-      if (p < i)
-        for (int q = p; q < i; q++) sacc += Ts[p + q * KMAX] * tmp[q];
-      __syncthreads();
-      if (p < i) Ts[p + i * KMAX] = sacc;
-      if (p == i) Ts[i + i * KMAX] = ti;
-    }
-    __syncthreads();
-  }
+  larft_recurrence(Ss, Ts, ts, tmp, kk);
   for (int e = tid; e < KMAX * KMAX; e += 256) T[e] = Ts[e];
 }
 

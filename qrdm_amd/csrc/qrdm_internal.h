@@ -190,6 +190,14 @@ void launch_gemm1_prec(int prec, double* A, int lda, int j, int m, int c0, int n
 void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n,
                        const double* V, int ldv, const double* W, hipStream_t s,
                        const DevState* st);
+// Batched DM-QR (batched.hip): one persistent launch of grid workgroups of
+// 64 * waves threads (waves 4 or 8); items points to device BatchItem
+// records (qrdm_batch_item), workgroup w runs sched[sched_off[w]..sched_off[w+1]).
+constexpr int BATCH_NMAX = 512;
+void launch_dm_batch(int waves, int grid, const void* items, const int* sched,
+                     const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
+                     int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
+                     double tol3z, hipStream_t s);
 // launch_larfb_w's two launches: the partial products Wp = V^T C per row
 // slab, then W = op(T) sum Wp.
 void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
