@@ -64,8 +64,29 @@ typedef struct {
                          precision (C itself stays FP64 in memory); the panel,
                          the norm downdate/recompute and the cosine test stay
                          FP64 */
-  const void *policy; /* reserved, must be NULL */
+  const void *policy; /* NULL, or a qrdm_policy in DEVICE memory: the learned
+                         network proposes each step's tau (candidate count)
+                         and delta (cosine bound); the panel's truncation keeps
+                         thres[1] */
 } qrdm_opts;
+
+/*
+ * The learned (tau, delta) proposal: a perceptron 34 -> 64 -> 64 -> 2 (ReLU,
+ * sigmoid outputs) with bfloat16 weights and FP32 accumulation.  Inputs: a
+ * 32-bin histogram of the remaining partial norms divided by their maximum
+ * (bin floor(32 r), clamped to 31; counts divided by n - j), the step
+ * fraction j / min(m, n), and min((m - j) / (n - j), 8) / 8.  Outputs y0, y1
+ * map to tau = tau_lo + (tau_hi - tau_lo) y0, delta = delta_lo +
+ * (delta_hi - delta_lo) y1; a non-finite or out-of-range proposal falls back
+ * to thres.  No counterpart in the reference.
+ */
+typedef struct {
+  unsigned short w1[64 * 34]; /* [out][in], bfloat16 bit patterns */
+  unsigned short w2[64 * 64];
+  unsigned short w3[2 * 64];
+  float b1[64], b2[64], b3[2];
+  float tau_lo, tau_hi, delta_lo, delta_hi;
+} qrdm_policy;
 
 /* Per-step record of the DM loop (debugging and step-by-step parity). */
 typedef struct {
@@ -78,6 +99,8 @@ typedef struct {
   int panel;     /* 1: Gram-based panel, 0: column-by-column panel */
   double maxnrm; /* largest partial norm after the step's downdate */
   int sel[64];   /* accepted columns relative to j, in acceptance order */
+  double tau;    /* the step's tau and delta (thres, or the learned proposal) */
+  double delta;
 } qrdm_step;
 
 /*
@@ -111,7 +134,7 @@ int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double *d_a, int lda, 
  * d_a + a_off; its tau (min(m, n)) at d_tau + tau_off; its jpvt and ncols
  * (n entries each, jpvt 1-based on exit) at d_jpvt + col_off and
  * d_ncols + col_off; its info (0, or -13 for a non-finite entry, the matrix
- * then untouched) at d_info[i].  All device pointers.  Limits: 1 <= n <= 512,
+ * then untouched) at d_info[i].  All device pointers.  Limits: 1 <= n <= 1024,
  * m >= 1, lda >= m.
  */
 typedef struct {
@@ -130,6 +153,20 @@ qrdm_batch_plan *qrdm_batch_plan_create(int count, const qrdm_batch_item *items,
 void qrdm_batch_plan_destroy(qrdm_batch_plan *plan);
 /* out[0..4) = {waves per workgroup, workgroups, nb, timed candidates}. */
 int qrdm_batch_plan_query(const qrdm_batch_plan *plan, int *out, int nout);
+/* Uses a learned (tau, delta) network (qrdm_policy in device memory, or
+ * NULL for thres) in every factorisation of the batch. */
+int qrdm_batch_plan_set_policy(qrdm_batch_plan *plan, const void *d_policy);
+/* One eager run of the batch that also writes a training record per DM
+ * step: 40 floats = the 34 network inputs, the label (tau, delta) scaled to
+ * [0, 1] over ranges = {tau_lo, tau_hi, delta_lo, delta_hi}, the block the
+ * defaults reduce, the label's block, j, n - j.  The label is the point of a
+ * (tau, delta) grid whose greedy acceptance yields the largest block that
+ * the panel's truncation test (default tau) would keep, judged on the
+ * candidates' Cholesky factor.  Up to cap records go to d_records; *d_count
+ * (device, zero on entry) counts all steps. */
+int qrdm_batch_trace(qrdm_batch_plan *plan, double *d_a, double *d_tau, int *d_jpvt,
+                     int *d_ncols, int *d_info, const double *thres, float *d_records, int cap,
+                     unsigned *d_count, const float *ranges, void *stream);
 /* Sets the launch shape by hand (waves per workgroup, 4 or 8). */
 int qrdm_batch_plan_set_shape(qrdm_batch_plan *plan, int waves);
 /* Times the launch shapes (4 or 8 waves per workgroup; and nb when the plan
@@ -145,6 +182,30 @@ int qrdm_batch_autotune(qrdm_batch_plan *plan, const double *d_a, const double *
 int qrdm_dgeqrdm_batched(qrdm_batch_plan *plan, double *d_a, double *d_tau, int *d_jpvt,
                          int *d_ncols, int *d_info, const double *thres, int stop_mode,
                          int use_graph, void *stream);
+
+/*
+ * Training of the proposal network on the device (no counterpart in the
+ * reference).  theta: the FP32 master parameters, W1 (64 x 34, row-major),
+ * b1, W2 (64 x 64), b2, W3 (2 x 64), b3; the forward pass uses their
+ * bfloat16 roundings.  x: ns x 34 inputs, y: ns x 2 targets in [0, 1];
+ * loss = (1/ns) sum (sigmoid(z) - y)^2.  All pointers device memory.
+ * qrdm_policy_sizes: out = {parameters, sizeof(qrdm_policy), floats per
+ * training record}.
+ */
+int qrdm_policy_sizes(int *out, int nout);
+/* d_grad = dloss/dtheta, d_loss[0] = loss. */
+int qrdm_policy_grad(const float *d_theta, const float *d_x, const float *d_y, int ns,
+                     float *d_grad, float *d_loss, void *stream);
+/* One Adam step (beta 0.9, 0.999, eps 1e-8, step number t >= 1) on the
+ * minibatch; d_m1, d_m2 the moments (zero before the first step). */
+int qrdm_policy_adam(float *d_theta, float *d_m1, float *d_m2, const float *d_x,
+                     const float *d_y, int ns, float lr, int t, float *d_loss, void *stream);
+/* theta -> qrdm_policy (bfloat16 weights); ranges (host) = {tau_lo, tau_hi,
+ * delta_lo, delta_hi}. */
+int qrdm_policy_pack(const float *d_theta, const float *ranges, void *d_net, void *stream);
+/* d_out[2 s + o] = the network's sigmoid outputs for the inputs d_x[34 s..]. */
+int qrdm_policy_predict(const void *d_net, const float *d_x, int ns, float *d_out,
+                        void *stream);
 
 /*
  * Unpivoted Householder QR on the device (LAPACK dgeqrf semantics; the

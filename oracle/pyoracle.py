@@ -42,6 +42,11 @@ def lib():
         _lib.oracle_dgeqrdm_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                            ctypes.c_int, _ip, _dp, _ip, _dp, ctypes.c_int,
                                            ctypes.POINTER(Step), ctypes.c_int, _ip]
+        _lib.oracle_dgeqrdm_sched.restype = ctypes.c_int
+        _lib.oracle_dgeqrdm_sched.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
+                                              ctypes.c_int, _ip, _dp, _ip, _dp, ctypes.c_int,
+                                              _dp, _dp, ctypes.c_int, ctypes.POINTER(Step),
+                                              ctypes.c_int, _ip]
         _lib.oracle_nrm2.restype = ctypes.c_double
         _lib.oracle_nrm2.argtypes = [ctypes.c_int, _dp]
         for name in ("oracle_ref_dgeqrdm",):
@@ -99,7 +104,7 @@ def _prep(A, jpvt, ncols_mode, n):
 
 
 def dgeqrdm(A, delta=0.9, tau=0.15, nb=64, stop=0, eta=0.0, jpvt=None, trace=False,
-            impl="oracle", layout=102):
+            impl="oracle", layout=102, sched=None):
     """Runs DM-QR on a copy of the column-major matrix A.
 
     impl="oracle" runs the C restatement, impl="ref" the compiled reference.
@@ -116,8 +121,16 @@ def dgeqrdm(A, delta=0.9, tau=0.15, nb=64, stop=0, eta=0.0, jpvt=None, trace=Fal
         cap = n + 1
         tr = (Step * cap)()
         ns = ctypes.c_int(0)
-        info = L.oracle_dgeqrdm_ex(layout, m, n, ptr(a, _dp), m, ptr(jp, _ip), ptr(t, _dp),
-                                   ptr(nc, _ip), ptr(th, _dp), nb, tr, cap, ctypes.byref(ns))
+        if sched is None:
+            info = L.oracle_dgeqrdm_ex(layout, m, n, ptr(a, _dp), m, ptr(jp, _ip), ptr(t, _dp),
+                                       ptr(nc, _ip), ptr(th, _dp), nb, tr, cap, ctypes.byref(ns))
+        else:
+            st_ = np.ascontiguousarray(sched[0], np.float64)
+            sd_ = np.ascontiguousarray(sched[1], np.float64)
+            info = L.oracle_dgeqrdm_sched(layout, m, n, ptr(a, _dp), m, ptr(jp, _ip),
+                                          ptr(t, _dp), ptr(nc, _ip), ptr(th, _dp), nb,
+                                          ptr(st_, _dp), ptr(sd_, _dp), len(st_), tr, cap,
+                                          ctypes.byref(ns))
         nsteps = ns.value
         if trace:
             steps = [dict(j=s.j, kmax=s.kmax, nc=s.nc, fjb=s.fjb, fjb_cmp=s.fjb_cmp,

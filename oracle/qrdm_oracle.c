@@ -488,9 +488,29 @@ void o_dormqr_left(int trans, int m, int n, int k, double *a, int lda, const dou
  * 102 is coherent (101 passes validation there, and here).  trace (optional)
  * receives one record per step, up to ntrace records; *nsteps the count.
  */
+/*
+ * oracle_dgeqrdm_sched: the same loop where step it selects with
+ * sched_tau[it], sched_delta[it] (it < nsched; thres afterwards) in place of
+ * thres[1], thres[0]; the panel's truncation keeps thres[1].  This is how
+ * the device path runs with a learned (tau, delta) proposal, which records
+ * the parameters of every step.
+ */
+int oracle_dgeqrdm_sched(int matrix_layout, int m, int n, double *a, int lda, int *jpvt,
+                         double *tau, int *ncols, const double *thres, int nb,
+                         const double *sched_tau, const double *sched_delta, int nsched,
+                         oracle_step *trace, int ntrace, int *nsteps);
+
 int oracle_dgeqrdm_ex(int matrix_layout, int m, int n, double *a, int lda, int *jpvt,
                       double *tau, int *ncols, const double *thres, int nb,
                       oracle_step *trace, int ntrace, int *nsteps) {
+  return oracle_dgeqrdm_sched(matrix_layout, m, n, a, lda, jpvt, tau, ncols, thres, nb, NULL,
+                              NULL, 0, trace, ntrace, nsteps);
+}
+
+int oracle_dgeqrdm_sched(int matrix_layout, int m, int n, double *a, int lda, int *jpvt,
+                         double *tau, int *ncols, const double *thres, int nb,
+                         const double *sched_tau, const double *sched_delta, int nsched,
+                         oracle_step *trace, int ntrace, int *nsteps) {
   double delta = thres[0], tau_dm = thres[1], eta = 0.0;
   int enable_stop = 0;
   double eps = O_EPS;
@@ -573,7 +593,9 @@ int oracle_dgeqrdm_ex(int matrix_layout, int m, int n, double *a, int lda, int *
         tr->j = j;
         tr->kmax = kmax;
       }
-      o_dm_perm(m, nsub, j, &work[j], a + (size_t)j * lda, lda, &jpvt[j], tau_dm, delta,
+      const double tau_s = (sched_tau && it < nsched) ? sched_tau[it] : tau_dm;
+      const double delta_s = (sched_delta && it < nsched) ? sched_delta[it] : delta;
+      o_dm_perm(m, nsub, j, &work[j], a + (size_t)j * lda, lda, &jpvt[j], tau_s, delta_s,
                 kmax, idx_sel, &fjb, mark, idxtmp, as, cosmat, keys, tr);
       double *ajj = a + j + (size_t)j * lda;
       o_dgeqr2_dm(msub, fjb, ajj, lda, &tau[j], w1, tau_dm, &fjb_cmp);

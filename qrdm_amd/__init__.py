@@ -37,7 +37,8 @@ class Step(ctypes.Structure):
     """qrdm_step of include/qrdm_amd.h."""
     _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
                 ("fjb", ctypes.c_int), ("fjb_cmp", ctypes.c_int), ("nswap", ctypes.c_int),
-                ("panel", ctypes.c_int), ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64)]
+                ("panel", ctypes.c_int), ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64),
+                ("tau", ctypes.c_double), ("delta", ctypes.c_double)]
 
 
 # Every entry point include/qrdm_amd.h declares.
@@ -46,7 +47,9 @@ EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrd
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
            "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
            "qrdm_profile_read", "qrdm_batch_plan_create", "qrdm_batch_plan_destroy",
-           "qrdm_batch_plan_query", "qrdm_batch_plan_set_shape", "qrdm_batch_autotune", "qrdm_dgeqrdm_batched",
+           "qrdm_batch_plan_query", "qrdm_batch_plan_set_shape", "qrdm_batch_autotune",
+           "qrdm_batch_plan_set_policy", "qrdm_batch_trace", "qrdm_policy_sizes",
+           "qrdm_policy_grad", "qrdm_policy_adam", "qrdm_policy_pack", "qrdm_policy_predict", "qrdm_dgeqrdm_batched",
            "qrdm_last_error", "qrdm_version")
 
 PROFILE_CLASSES = ("larfb_w", "norm_update", "select", "panel", "permute", "init_norms",
@@ -78,6 +81,15 @@ def lib():
         L.qrdm_batch_plan_destroy.argtypes = [V]
         L.qrdm_batch_plan_query.argtypes = [V, _ip, ctypes.c_int]
         L.qrdm_batch_plan_set_shape.argtypes = [V, ctypes.c_int]
+        L.qrdm_batch_plan_set_policy.argtypes = [V, V]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.qrdm_batch_trace.argtypes = [V, V, V, V, V, V, _dp, V, ctypes.c_int, V, _fp, V]
+        L.qrdm_policy_sizes.argtypes = [_ip, ctypes.c_int]
+        L.qrdm_policy_grad.argtypes = [V, V, V, ctypes.c_int, V, V, V]
+        L.qrdm_policy_adam.argtypes = [V, V, V, V, V, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                       V, V]
+        L.qrdm_policy_pack.argtypes = [V, _fp, V, V]
+        L.qrdm_policy_predict.argtypes = [V, V, ctypes.c_int, V, V]
         L.qrdm_batch_autotune.argtypes = [V, V, _dp, V]
         L.qrdm_dgeqrdm_batched.argtypes = [V, V, V, V, V, V, _dp, ctypes.c_int, ctypes.c_int, V]
         L.qrdm_dgeqrf_dev.argtypes = [ctypes.c_int, ctypes.c_int, V, ctypes.c_int, V, V]
@@ -102,7 +114,7 @@ def last_error():
 
 
 def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
-                layout=102, trailing="fp64"):
+                layout=102, trailing="fp64", policy=None):
     """DM-QR in place of a device matrix.
 
     a: contiguous float64 device tensor of shape (n, lda) -- its memory is the
@@ -111,7 +123,8 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
     thres: float64 numpy array [delta, tau(, eta)].  stream: hipStream_t as
     an int (torch.cuda.current_stream().cuda_stream), None = default stream.
     trailing: precision of the trailing update's products, "fp64" (the
-    plain path), "fp32" or "bf16x3" (qrdm_dgeqrdm_dev_opt).
+    plain path), "fp32" or "bf16x3" (qrdm_dgeqrdm_dev_opt).  policy: a
+    device tensor holding a qrdm_policy (qrdm_amd.policy), or None.
     Returns (info, steps), steps = per-step records when trace is set.
     """
     import numpy as np
@@ -128,17 +141,18 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
             thres.ctypes.data_as(_dp), nb, s)
     if trailing not in TRAILING:
         raise ValueError("trailing must be one of %s" % ", ".join(TRAILING))
-    if trace or trailing != "fp64":
+    if trace or trailing != "fp64" or policy is not None:
         cap = max(1, min(m, n)) if trace else 0
         tr = (Step * max(cap, 1))()
         ns = ctypes.c_int(0)
-        opts = Opts(TRAILING[trailing], None)
+        opts = Opts(TRAILING[trailing], None if policy is None else policy.data_ptr())
         info = L.qrdm_dgeqrdm_dev_opt(*args[:-1], ctypes.byref(opts), s,
                                       tr if trace else None, cap, ctypes.byref(ns))
         if not trace:
             return info, None
         steps = [dict(j=t.j, kmax=t.kmax, nc=t.nc, fjb=t.fjb, fjb_cmp=t.fjb_cmp, nswap=t.nswap,
-                      panel=t.panel, maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]))
+                      panel=t.panel, maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]),
+                      tau=t.tau, delta=t.delta)
                  for t in tr[:ns.value]]
         return info, steps
     return L.qrdm_dgeqrdm_dev(*args), None

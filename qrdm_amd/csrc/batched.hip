@@ -25,6 +25,11 @@ struct BatchItem {
 struct BatchParams {
   double delta, tau_dm, eta3, tol3z;
   int nb, stop_mode;
+  const PolicyNet* policy;  // learned (tau, delta) per step, or nullptr
+  float* trace;             // per-step training records (POL_REC floats), or nullptr
+  unsigned* trace_count;    // records written (may exceed trace_cap)
+  int trace_cap;
+  float tau_lo, tau_hi, delta_lo, delta_hi;  // label normalisation
 };
 
 template <int NW>
@@ -39,7 +44,9 @@ struct BatchLds {
   double red[16];
   int perm[4 * KMAX + KMAX];
   DevState st;
-  int s_int[4];
+  float feat[POL_IN];
+  float hbuf[POL_H];
+  int hist[POL_BINS];
 };
 
 // Unit lower trapezoidal V of the panel at column j: V(row, c) for c < kk.
@@ -281,6 +288,32 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
           L.st.kmax = kmax;
           L.st.j = j;
           L.st.halt = 0;
+          L.st.policy_on = 0;
+          L.st.tau_step = P.tau_dm;
+          L.st.delta_step = P.delta;
+        }
+        // the learned proposal (k_policy's arithmetic), and the inputs of a
+        // training record
+        if (P.policy || P.trace) {
+          wave_lds_sync();
+          policy_features(L.work + j, nsub, L.st.cand_val[0], j, m, n, L.feat, L.hist);
+        }
+        if (P.policy) {
+          const float2 y = policy_forward(P.policy, L.feat, L.hbuf);
+          double tp = P.policy->tau_lo + (P.policy->tau_hi - P.policy->tau_lo) * (double)y.x;
+          double dp = P.policy->delta_lo + (P.policy->delta_hi - P.policy->delta_lo) * (double)y.y;
+          if (!(tp >= 0.0 && tp <= 1.0 && dp >= 0.0 && dp <= 1.0)) {
+            tp = P.tau_dm;
+            dp = P.delta;
+          }
+          const double lim2 = tp * L.lv[0][0];
+          const int nc2 = __popcll(__ballot(lane < kmax && cv > lim2));
+          if (lane == 0) {
+            L.st.nc = nc2;
+            L.st.policy_on = 1;
+            L.st.tau_step = tp;
+            L.st.delta_step = dp;
+          }
         }
       }
       __syncthreads();
@@ -288,10 +321,21 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
     const int nc = L.st.nc;
     // --- cosines (src/dgeqrdm_work.c:365-379): Gram of the candidate
     // columns over rows j..m, scaled by the reciprocal norms in select_wave
-    if (nc > 1) {
+    // (over all kmax candidates when a training record scores other tau)
+    const int ng = P.trace ? kmax : nc;
+    if (ng > 1) {
       const DevState& st = L.st;
-      bt_gram<NW>(j, m, nc,
+      bt_gram<NW>(j, m, ng,
                   [&](int c, int r) { return a[(size_t)(j + st.cand_idx[c]) * lda + r]; }, L);
+    }
+    if (P.trace && wave == 0) {
+      unsigned slot = 0;
+      if (lane == 0) slot = atomicAdd(P.trace_count, 1u);
+      slot = (unsigned)__builtin_amdgcn_readfirstlane((int)slot);
+      if (slot < (unsigned)P.trace_cap)
+        label_record(L.R1, L.st.cand_val, kmax, P.tau_dm, P.delta, L.Ts, L.feat, j, nsub,
+                     P.tau_lo, P.tau_hi, P.delta_lo, P.delta_hi,
+                     P.trace + (size_t)slot * POL_REC);
     }
     // --- greedy acceptance and the permute_marked plan
     if (wave == 0) select_wave(L.R1, P.delta, m, n, INT_MAX, 1, L.perm, &L.st);
@@ -416,8 +460,12 @@ __global__ __launch_bounds__(64 * NW) void k_dm_batch(const BatchItem* __restric
 void launch_dm_batch(int waves, int grid, const void* items, const int* sched,
                      const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
-                     double tol3z, hipStream_t s) {
-  BatchParams P{delta, tau_dm, eta3, tol3z, nb, stop_mode};
+                     double tol3z, const void* policy, float* trace, unsigned* trace_count,
+                     int trace_cap, const float* ranges, hipStream_t s) {
+  BatchParams P{delta, tau_dm, eta3, tol3z, nb, stop_mode,
+                static_cast<const PolicyNet*>(policy), trace, trace_count, trace_cap,
+                ranges ? ranges[0] : 0.f, ranges ? ranges[1] : 1.f,
+                ranges ? ranges[2] : 0.f, ranges ? ranges[3] : 1.f};
   const BatchItem* it = static_cast<const BatchItem*>(items);
   if (waves == 8)
     hipLaunchKernelGGL(k_dm_batch<8>, dim3(grid), dim3(512), 0, s, it, sched, sched_off, A, tau,

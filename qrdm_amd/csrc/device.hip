@@ -1,8 +1,10 @@
 // The device code of libqrdm_amd.so as one translation unit: the kernels of
 // panel.hip combine stages defined in kernels.hip (one launch runs several
 // steps of a DM iteration); mixed.hip adds the reduced-precision trailing
-// update, batched.hip the persistent many-matrix kernel.
+// update, learned.hip the (tau, delta) network, batched.hip the persistent
+// many-matrix kernel.
 #include "kernels.hip"
 #include "panel.hip"
 #include "mixed.hip"
+#include "learned.hip"
 #include "batched.hip"

@@ -217,6 +217,7 @@ struct Args {
   int ntrace;
   int* nsteps;
   int trail = TRAIL_FP64;  // precision of the trailing update's products
+  const void* policy = nullptr;  // learned (tau, delta) network on the device, or null
 };
 
 int check_args(const Args& x) {
@@ -346,7 +347,7 @@ int run_dm(const Args& x) {
     const bool force_exact = pe && strcmp(pe, "exact") == 0;
     const int fast_rows = force_exact ? INT_MAX : kFastPanelRows;
     HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
-    launch_dm_begin(B.st, nfxd, minmn, s);
+    launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, s);
     const int batch = x.trace ? 1 : kBatch;
     int j0 = nfxd, it = 0;
     // the exact panel and dlarft of the step at column jl (RUN_EXACT: they
@@ -381,6 +382,8 @@ int run_dm(const Args& x) {
       int ps = g_prof.begin(PC_SELECT, s);
       launch_topk(B.work, jl, m, n, nb, tau_dm, B.topk_val, B.topk_idx, B.tickets, B.st, B.ncols,
                   enable_stop, eta, s);
+      // the learned proposal replaces the step's candidate count and delta
+      if (x.policy) launch_policy(B.work, m, n, x.policy, B.st, s);
       // candidate Gram over rows j..m (cosines need it only when nc > 1;
       // the kernel reads nc from the state and zero-fills beyond it), then
       // the greedy acceptance and the permutation plan
@@ -464,6 +467,8 @@ int run_dm(const Args& x) {
         r.nswap = l.nswap;
         r.panel = l.panel;
         r.maxnrm = l.maxnrm;
+        r.tau = l.tau;
+        r.delta = l.delta;
         for (int t = 0; t < l.fjb && t < 64; t++) r.sel[t] = h->sel[t];
       }
       it = h->it;
@@ -617,6 +622,7 @@ struct qrdm_batch_plan {
   } key{};
   double tuned_ms[8] = {};
   int tuned_n = 0;
+  const void* policy = nullptr;  // learned (tau, delta) network on the device, or null
 };
 
 namespace {
@@ -669,9 +675,12 @@ void batch_release_graph(qrdm_batch_plan* p) {
 }
 
 void batch_launch(qrdm_batch_plan* p, int waves, int nb, double* a, double* tau, int* jp,
-                  int* nc, int* info, const double* th, int stop, hipStream_t s) {
+                  int* nc, int* info, const double* th, int stop, hipStream_t s,
+                  float* trace = nullptr, unsigned* trace_count = nullptr, int trace_cap = 0,
+                  const float* ranges = nullptr) {
   launch_dm_batch(waves, p->grid, p->d_items, p->d_sched, p->d_off, a, tau, jp, nc, info, nb,
-                  th[0], th[1], stop, th[2], sqrt(kEps), s);
+                  th[0], th[1], stop, th[2], sqrt(kEps), p->policy, trace, trace_count,
+                  trace_cap, ranges, s);
   HIPCHK(hipGetLastError());
 }
 
@@ -749,6 +758,7 @@ int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double* d_a, int lda, 
       return -1;
     }
     x.trail = opts->trailing;
+    x.policy = opts->policy;
   }
   return guarded(x);
 }
@@ -874,7 +884,7 @@ qrdm_batch_plan* qrdm_batch_plan_create(int count, const qrdm_batch_item* items,
     if (t.m < 1 || t.n < 1 || t.n > BATCH_NMAX || t.lda < t.m || t.a_off < 0 || t.tau_off < 0 ||
         t.col_off < 0) {
       g_err = "batch: item " + std::to_string(i) +
-              " needs m >= 1, 1 <= n <= 512, lda >= m and non-negative offsets";
+              " needs m >= 1, 1 <= n <= 1024, lda >= m and non-negative offsets";
       delete p;
       return nullptr;
     }
@@ -921,6 +931,28 @@ int qrdm_batch_plan_query(const qrdm_batch_plan* p, int* out, int nout) {
                           (double)p->tuned_n};
   for (int i = 0; i < nout && i < 4; i++) out[i] = (int)vals[i];
   return 4;
+}
+
+int qrdm_batch_plan_set_policy(qrdm_batch_plan* p, const void* d_policy) {
+  if (!p) return -1;
+  p->policy = d_policy;
+  batch_release_graph(p);
+  return 0;
+}
+
+int qrdm_batch_trace(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_jpvt, int* d_ncols,
+                     int* d_info, const double* thres, float* d_records, int cap,
+                     unsigned* d_count, const float* ranges, void* stream) {
+  if (!p || !d_a || !d_tau || !d_jpvt || !d_ncols || !d_info || !thres || !d_records ||
+      !d_count || !ranges || cap < 0)
+    return -1;
+  return guarded_call([&] {
+    hipStream_t s = (hipStream_t)stream;
+    const double th[3] = {thres[0], thres[1], 0.0};
+    batch_launch(p, p->waves, p->nb_run, d_a, d_tau, d_jpvt, d_ncols, d_info, th, 0, s,
+                 d_records, d_count, cap, ranges);
+    return 0;
+  });
 }
 
 int qrdm_batch_plan_set_shape(qrdm_batch_plan* p, int waves) {
@@ -1013,12 +1045,78 @@ int qrdm_dgeqrdm_batched(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_
       HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
       launch_dm_batch(p->waves, p->grid, p->d_items, p->d_sched, p->d_off, d_a, d_tau, d_jpvt,
                       d_ncols, d_info, p->nb_run, th[0], th[1], stop_mode, th[2], sqrt(kEps),
-                      p->cap);
+                      p->policy, nullptr, nullptr, 0, nullptr, p->cap);
       HIPCHK(hipStreamEndCapture(p->cap, &p->graph));
       HIPCHK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
       p->key = k;
     }
     HIPCHK(hipGraphLaunch(p->exec, s));
+    return 0;
+  });
+}
+
+int qrdm_policy_sizes(int* out, int nout) {
+  const int v[3] = {policy_nparam(), (int)policy_net_bytes(), POLICY_REC};
+  for (int i = 0; i < nout && i < 3; i++) out[i] = v[i];
+  return 3;
+}
+
+static float* policy_partials_buffer() {
+  static float* buf[64] = {};
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (!buf[dev & 63])
+    HIPCHK(hipMalloc(&buf[dev & 63], sizeof(float) * ((size_t)policy_partials() *
+                                                      (policy_nparam() + 1))));
+  return buf[dev & 63];
+}
+
+int qrdm_policy_grad(const float* d_theta, const float* d_x, const float* d_y, int ns,
+                     float* d_grad, float* d_loss, void* stream) {
+  if (!d_theta || !d_x || !d_y || !d_grad || !d_loss || ns <= 0) return -1;
+  return guarded_call([&] {
+    hipStream_t s = (hipStream_t)stream;
+    float* part = policy_partials_buffer();
+    float* lpart = part + (size_t)policy_partials() * policy_nparam();
+    launch_policy_grad(d_theta, d_x, d_y, ns, part, lpart, s);
+    launch_policy_update(nullptr, nullptr, nullptr, part, lpart, d_grad, d_loss, 0, 0.f, 0.f,
+                         0.f, 0.f, 1, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+  });
+}
+
+int qrdm_policy_adam(float* d_theta, float* d_m1, float* d_m2, const float* d_x,
+                     const float* d_y, int ns, float lr, int t, float* d_loss, void* stream) {
+  if (!d_theta || !d_m1 || !d_m2 || !d_x || !d_y || !d_loss || ns <= 0 || t < 1) return -1;
+  return guarded_call([&] {
+    hipStream_t s = (hipStream_t)stream;
+    float* part = policy_partials_buffer();
+    float* lpart = part + (size_t)policy_partials() * policy_nparam();
+    launch_policy_grad(d_theta, d_x, d_y, ns, part, lpart, s);
+    launch_policy_update(d_theta, d_m1, d_m2, part, lpart, nullptr, d_loss, 1, lr, 0.9f,
+                         0.999f, 1e-8f, t, s);
+    HIPCHK(hipGetLastError());
+    return 0;
+  });
+}
+
+int qrdm_policy_pack(const float* d_theta, const float* ranges, void* d_net, void* stream) {
+  if (!d_theta || !ranges || !d_net) return -1;
+  return guarded_call([&] {
+    launch_policy_pack(d_theta, d_net, ranges[0], ranges[1], ranges[2], ranges[3],
+                       (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+  });
+}
+
+int qrdm_policy_predict(const void* d_net, const float* d_x, int ns, float* d_out,
+                        void* stream) {
+  if (!d_net || !d_x || !d_out || ns <= 0) return -1;
+  return guarded_call([&] {
+    launch_policy_predict(d_net, d_x, ns, d_out, (hipStream_t)stream);
+    HIPCHK(hipGetLastError());
     return 0;
   });
 }

@@ -340,6 +340,8 @@ __device__ void step_end_apply(DevState* st, int* ncols_dev, int m, int n, int e
   r.nswap = st->nswap;
   r.panel = (st->fast && !(st->flags & 2)) ? 1 : 0;
   r.maxnrm = maxnrm;
+  r.tau = st->tau_step;
+  r.delta = st->delta_step;
   const double nct = (double)(n - st->c0);
   StepWork& w = st->work;
   w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
@@ -605,6 +607,7 @@ __device__ void select_wave(const double* Gs, double delta, int m, int n, int fa
   const int t = threadIdx.x;
   const int j = st->j, nc = st->nc, kmax = st->kmax;
   const int nsub = n - j;
+  if (st->policy_on) delta = st->delta_step;
   // --- 1. greedy.  Lane t builds the mask of earlier candidates s whose
   // |cos(s, t)| >= delta; candidate i is accepted when its mask misses every
   // accepted candidate (max |cos| < delta) and fewer than k_max are accepted.
@@ -1535,7 +1538,10 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s, st, ncols_dev, m, n, enable_stop, eta);
 }
 
-__global__ void k_dm_begin(DevState* st, int j, int minmn) {
+__global__ void k_dm_begin(DevState* st, int j, int minmn, double tau, double delta) {
+  st->policy_on = 0;
+  st->tau_step = tau;
+  st->delta_step = delta;
   st->pend = 0;
   st->halt = 0;
   st->j = j;
@@ -1546,8 +1552,8 @@ __global__ void k_dm_begin(DevState* st, int j, int minmn) {
   st->work = StepWork{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 }
 
-void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s) {
-  hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(1), 0, s, st, j, minmn);
+void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, hipStream_t s) {
+  hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(1), 0, s, st, j, minmn, tau, delta);
 }
 
 // ---------------------------------------------------------------------------

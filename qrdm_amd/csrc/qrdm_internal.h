@@ -13,6 +13,7 @@ constexpr int GRAM_ROWS = 64;     // rows per workgroup of the Gram kernel
 struct StepRec {
   int j, kmax, nc, fjb, fjb_cmp, nswap, panel;
   double maxnrm;
+  double tau, delta;  // the DM parameters the step used
 };
 
 // Algorithmic work of the steps run so far, per kernel class (flops or HBM
@@ -50,6 +51,9 @@ struct DevState {
                // kernels return until the host resumes the step.  2: the step's
                // column moves still run (set before them); the next step's
                // top-k turns it into 1, which stops the moves too
+  int policy_on;      // 1: the step's tau and delta come from the learned policy
+  double tau_step;    // tau and delta of the current step (the defaults, or the
+  double delta_step;  // policy's proposal)
   StepRec last;
   StepWork work;
 };
@@ -193,11 +197,30 @@ void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n
 // Batched DM-QR (batched.hip): one persistent launch of grid workgroups of
 // 64 * waves threads (waves 4 or 8); items points to device BatchItem
 // records (qrdm_batch_item), workgroup w runs sched[sched_off[w]..sched_off[w+1]).
-constexpr int BATCH_NMAX = 512;
+// policy: a PolicyNet (learned.hip) proposing each step's (tau, delta), or
+// nullptr; trace: per-step training records, or nullptr (ranges: the label
+// normalisation tau_lo, tau_hi, delta_lo, delta_hi).
+constexpr int BATCH_NMAX = 1024;
 void launch_dm_batch(int waves, int grid, const void* items, const int* sched,
                      const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
-                     double tol3z, hipStream_t s);
+                     double tol3z, const void* policy, float* trace, unsigned* trace_count,
+                     int trace_cap, const float* ranges, hipStream_t s);
+// The learned (tau, delta) proposal (learned.hip).
+int policy_nparam();
+size_t policy_net_bytes();
+int policy_partials();
+constexpr int POLICY_REC = 40;  // floats per training record
+void launch_policy(const double* work, int m, int n, const void* net, DevState* st,
+                   hipStream_t s);
+void launch_policy_grad(const float* theta, const float* x, const float* y, int ns,
+                        float* partial, float* loss_partial, hipStream_t s);
+void launch_policy_update(float* theta, float* m1, float* m2, const float* partial,
+                          const float* loss_partial, float* grad, float* loss_out, int adam,
+                          float lr, float beta1, float beta2, float eps, int t, hipStream_t s);
+void launch_policy_pack(const float* theta, void* net, float tau_lo, float tau_hi,
+                        float delta_lo, float delta_hi, hipStream_t s);
+void launch_policy_predict(const void* net, const float* x, int ns, float* out, hipStream_t s);
 // launch_larfb_w's two launches: the partial products Wp = V^T C per row
 // slab, then W = op(T) sum Wp.
 void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
@@ -215,7 +238,7 @@ void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* 
 void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
                      hipStream_t s);
 // Start of the DM loop at column j.
-void launch_dm_begin(DevState* st, int j, int minmn, hipStream_t s);
+void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
 // Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST

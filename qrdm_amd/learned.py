@@ -1,0 +1,219 @@
+"""The learned (tau, delta) proposal: training-data generation from the
+method's own steps, training on the device, and the packed network.
+
+The network and its kernels live in libqrdm_amd.so (csrc/learned.hip,
+include/qrdm_amd.h: qrdm_policy_*, qrdm_batch_trace); this module drives
+them with torch device tensors as plumbing:
+
+  traces   the batched DM-QR run in trace mode over seeded matrices of three
+           spectrum families writes one record per DM step: the 34 inputs
+           and the label, the (tau, delta) of a grid whose greedy acceptance
+           gives the largest block the panel's truncation test (default tau)
+           keeps;
+  train    minibatch Adam on the device (qrdm_policy_adam), FP32 master
+           weights, bfloat16 forward;
+  policy   the packed qrdm_policy the DM loop reads (qrdm_opts.policy).
+
+Families (seeded): geometric s_i = 10^(-d i/(k-1)), d ~ U[4, 14]; step,
+three clusters at 1, 1e-4, 1e-9 (sizes from a uniform split, values
+jittered by 10%); Kahan-type, A = U K D with K = diag(sin(t)^i)(I - cos(t) N)
+(N strictly upper triangular ones), t = 1.2, D = diag(1 + 0.01 N(0,1)).
+U, V have orthonormal columns from two passes of Cholesky QR of seeded
+Gaussians (batched on the device).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import BatchPlan, lib, last_error
+
+RANGES = (0.02, 0.40, 0.75, 0.99)  # tau_lo, tau_hi, delta_lo, delta_hi
+FAMILIES = ("geometric", "step", "kahan")
+HERE = os.path.dirname(os.path.abspath(__file__))
+THETA_PATH = os.path.join(HERE, "policy_theta.npy")
+
+
+def sizes():
+    out = np.zeros(3, np.int32)
+    lib().qrdm_policy_sizes(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 3)
+    return dict(nparam=int(out[0]), net_bytes=int(out[1]), rec=int(out[2]))
+
+
+def _orthonormal(torch, g):
+    """Orthonormal columns of the batched (b, m, k) Gaussian g: Cholesky QR,
+    twice."""
+    q = g
+    for _ in range(2):
+        gram = torch.bmm(q.transpose(1, 2).contiguous(), q)
+        r = torch.linalg.cholesky(gram, upper=True)
+        q = torch.linalg.solve_triangular(r, q, upper=True, left=False)
+    return q
+
+
+def _spectrum(family, k, rng):
+    if family == "geometric":
+        d = rng.uniform(4.0, 14.0)
+        return 10.0 ** (-d * np.arange(k) / max(k - 1, 1))
+    if family == "step":
+        cuts = np.sort(rng.integers(1, k, 2))
+        s = np.empty(k)
+        s[:cuts[0]] = 1.0
+        s[cuts[0]:cuts[1]] = 1e-4
+        s[cuts[1]:] = 1e-9
+        s *= 1.0 + 0.1 * rng.uniform(-1, 1, k)
+        return np.sort(s)[::-1]
+    raise ValueError(family)
+
+
+def _kahan(torch, n, theta=1.2):
+    sn, cs = np.sin(theta), np.cos(theta)
+    k = -cs * np.triu(np.ones((n, n)), 1) + np.eye(n)
+    k = (sn ** np.arange(n))[:, None] * k
+    return torch.from_numpy(k).cuda()
+
+
+def family_batch(torch, family, count, m, n, seed):
+    """(count, m, n) float64 device tensor of the family, and the spectra
+    (None for Kahan-type, whose singular values are not set by
+    construction)."""
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    rng = np.random.default_rng(seed + 7)
+    k = min(m, n)
+    u = _orthonormal(torch, torch.randn(count, m, k, generator=gen, device="cuda",
+                                        dtype=torch.float64))
+    if family == "kahan":
+        kk = _kahan(torch, n)
+        d = 1.0 + 0.01 * torch.randn(count, 1, n, generator=gen, device="cuda",
+                                     dtype=torch.float64)
+        a = torch.bmm(u, (kk[:k, :].unsqueeze(0) * d).contiguous())
+        return a, None
+    v = _orthonormal(torch, torch.randn(count, n, k, generator=gen, device="cuda",
+                                        dtype=torch.float64))
+    spectra = np.stack([_spectrum(family, k, rng) for _ in range(count)])
+    s = torch.from_numpy(spectra).cuda().unsqueeze(1)
+    a = torch.bmm((u * s).contiguous(), v.transpose(1, 2).contiguous())
+    return a, spectra
+
+
+def collect_traces(torch, count_per_family, m=2048, n=1024, seed=0, chunk=256,
+                   thres=(0.9, 0.15), log=None):
+    """Training records (N x 40 float32, host) from the method's steps on
+    count_per_family matrices of each family."""
+    recs = []
+    rb = sizes()["rec"]
+    ranges = np.array(RANGES, np.float32)
+    th = np.array([thres[0], thres[1], 0.0])
+    for fi, fam in enumerate(FAMILIES):
+        done = 0
+        while done < count_per_family:
+            b = min(chunk, count_per_family - done)
+            a, _ = family_batch(torch, fam, b, m, n, seed + 1000 * fi + done)
+            plan = BatchPlan([(m, n)] * b, nb=64)
+            buf = a.transpose(1, 2).contiguous().view(-1)  # column-major, packed
+            del a
+            tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
+            jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
+            nc = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
+            info = torch.zeros(b, dtype=torch.int32, device="cuda")
+            cap = b * n
+            out = torch.zeros(cap * rb, dtype=torch.float32, device="cuda")
+            cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+            r = lib().qrdm_batch_trace(plan._p, ctypes.c_void_p(buf.data_ptr()),
+                                       ctypes.c_void_p(tau.data_ptr()),
+                                       ctypes.c_void_p(jp.data_ptr()),
+                                       ctypes.c_void_p(nc.data_ptr()),
+                                       ctypes.c_void_p(info.data_ptr()),
+                                       th.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                       ctypes.c_void_p(out.data_ptr()), cap,
+                                       ctypes.c_void_p(cnt.data_ptr()),
+                                       ranges.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       None)
+            if r != 0:
+                raise RuntimeError("qrdm_batch_trace: %s" % last_error())
+            torch.cuda.synchronize()
+            k = min(int(cnt.item()), cap)
+            recs.append(out[:k * rb].view(k, rb).cpu().numpy())
+            plan.close()
+            done += b
+            if log:
+                log("traces: %s %d/%d matrices, %d records" % (fam, done, count_per_family, k))
+    return np.concatenate(recs)
+
+
+def train(torch, X, Y, epochs=20, batch=4096, lr=2e-3, seed=0, holdout=0.05, log=None):
+    """Minibatch Adam on the device (qrdm_policy_adam).  Returns the trained
+    parameters (float32 numpy) and per-epoch (train loss, held-out loss)."""
+    f32 = ctypes.c_void_p
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(X))
+    nh = int(len(X) * holdout)
+    Xd = torch.from_numpy(np.ascontiguousarray(X[order], np.float32)).cuda()
+    Yd = torch.from_numpy(np.ascontiguousarray(Y[order], np.float32)).cuda()
+    Xv, Yv, Xt, Yt = Xd[:nh], Yd[:nh], Xd[nh:], Yd[nh:]
+    th = torch.from_numpy(init_theta(seed)).cuda()
+    m1, m2 = torch.zeros_like(th), torch.zeros_like(th)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    grad = torch.zeros_like(th)
+    n, t, hist = len(Xt), 0, []
+    for ep in range(epochs):
+        perm = torch.from_numpy(rng.permutation(n)).cuda()
+        xs, ys = Xt[perm].contiguous(), Yt[perm].contiguous()
+        tot, cnt = 0.0, 0
+        for b0 in range(0, n - batch // 4, batch):
+            nb = min(batch, n - b0)
+            t += 1
+            r = lib().qrdm_policy_adam(f32(th.data_ptr()), f32(m1.data_ptr()), f32(m2.data_ptr()),
+                                       f32(xs[b0:].data_ptr()), f32(ys[b0:].data_ptr()), nb, lr, t,
+                                       f32(loss.data_ptr()), None)
+            if r != 0:
+                raise RuntimeError("qrdm_policy_adam: %s" % last_error())
+            tot += float(loss.item()) * nb
+            cnt += nb
+        vl = float("nan")
+        if nh > 0:
+            lib().qrdm_policy_grad(f32(th.data_ptr()), f32(Xv.data_ptr()), f32(Yv.data_ptr()), nh,
+                                   f32(grad.data_ptr()), f32(loss.data_ptr()), None)
+            vl = float(loss.item())
+        hist.append((tot / max(cnt, 1), vl))
+        if log:
+            log("train: epoch %d loss %.5f held-out %.5f" % (ep, hist[-1][0], vl))
+    return th.cpu().numpy(), hist
+
+
+def init_theta(seed):
+    """He-style initial parameters (float32 numpy)."""
+    nh, nin = 64, 34
+    rng = np.random.default_rng(seed)
+    p = sizes()["nparam"]
+    th = np.zeros(p, np.float32)
+    w1, b1 = 0, nh * nin
+    w2 = b1 + nh
+    b2 = w2 + nh * nh
+    w3 = b2 + nh
+    b3 = w3 + 2 * nh
+    th[w1:b1] = rng.normal(0, np.sqrt(2.0 / nin), nh * nin)
+    th[w2:b2] = rng.normal(0, np.sqrt(2.0 / nh), nh * nh)
+    th[w3:b3] = rng.normal(0, np.sqrt(1.0 / nh), 2 * nh)
+    return th
+
+
+def pack(torch, theta):
+    """The qrdm_policy of theta, as a device byte tensor."""
+    nb = sizes()["net_bytes"]
+    net = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+    th = torch.from_numpy(np.ascontiguousarray(theta, np.float32)).cuda()
+    rg = np.array(RANGES, np.float32)
+    r = lib().qrdm_policy_pack(ctypes.c_void_p(th.data_ptr()),
+                               rg.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               ctypes.c_void_p(net.data_ptr()), None)
+    if r != 0:
+        raise RuntimeError("qrdm_policy_pack: %s" % last_error())
+    torch.cuda.synchronize()
+    return net
+
+
+def load_policy(torch, path=THETA_PATH):
+    """The committed trained network, packed on the device."""
+    return pack(torch, np.load(path, allow_pickle=False))

@@ -190,7 +190,7 @@ def test_batched_nonfinite_and_bad_items(gpu):
     o = int(plan.a_off[2])
     assert np.array_equal(buf[o:o + m * n], src.cpu().numpy()[o:o + m * n], equal_nan=True)
     with pytest.raises(ValueError):
-        qrdm_amd.BatchPlan([(600, 513)])
+        qrdm_amd.BatchPlan([(1100, 1025)])
     with pytest.raises(ValueError):
         qrdm_amd.BatchPlan([(10, 5)], nb=65)
 
