@@ -292,28 +292,27 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
           L.st.tau_step = P.tau_dm;
           L.st.delta_step = P.delta;
         }
-        // the learned proposal (k_policy's arithmetic), and the inputs of a
-        // training record
-        if (P.policy || P.trace) {
-          wave_lds_sync();
-          policy_features(L.work + j, nsub, L.st.cand_val[0], j, m, n, L.feat, L.hist);
+      }
+      __syncthreads();
+      // the learned proposal (k_policy's arithmetic), and the inputs of a
+      // training record
+      if (P.policy || P.trace)
+        policy_features(L.work + j, nsub, L.st.cand_val[0], j, m, n, L.feat, L.hist);
+      if (P.policy && wave == 0) {
+        const double cv = L.lv[0][lane];
+        const float2 y = policy_forward(P.policy, L.feat, L.hbuf);
+        double tp = P.policy->tau_lo + (P.policy->tau_hi - P.policy->tau_lo) * (double)y.x;
+        double dp = P.policy->delta_lo + (P.policy->delta_hi - P.policy->delta_lo) * (double)y.y;
+        if (!(tp >= 0.0 && tp <= 1.0 && dp >= 0.0 && dp <= 1.0)) {
+          tp = P.tau_dm;
+          dp = P.delta;
         }
-        if (P.policy) {
-          const float2 y = policy_forward(P.policy, L.feat, L.hbuf);
-          double tp = P.policy->tau_lo + (P.policy->tau_hi - P.policy->tau_lo) * (double)y.x;
-          double dp = P.policy->delta_lo + (P.policy->delta_hi - P.policy->delta_lo) * (double)y.y;
-          if (!(tp >= 0.0 && tp <= 1.0 && dp >= 0.0 && dp <= 1.0)) {
-            tp = P.tau_dm;
-            dp = P.delta;
-          }
-          const double lim2 = tp * L.lv[0][0];
-          const int nc2 = __popcll(__ballot(lane < kmax && cv > lim2));
-          if (lane == 0) {
-            L.st.nc = nc2;
-            L.st.policy_on = 1;
-            L.st.tau_step = tp;
-            L.st.delta_step = dp;
-          }
+        const int nc2 = __popcll(__ballot(lane < kmax && cv > tp * L.lv[0][0]));
+        if (lane == 0) {
+          L.st.nc = nc2;
+          L.st.policy_on = 1;
+          L.st.tau_step = tp;
+          L.st.delta_step = dp;
         }
       }
       __syncthreads();

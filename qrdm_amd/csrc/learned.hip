@@ -49,28 +49,29 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-// The 34 inputs of step j into feat (LDS), by one wave; work holds the n - j
-// partial norms of the remaining columns, maxw their maximum; hist is LDS
-// scratch of POL_BINS ints.
+// The 34 inputs of step j into feat (LDS), by the whole workgroup (every
+// thread calls it; it ends with a barrier); work holds the n - j partial
+// norms of the remaining columns, maxw their maximum; hist is LDS scratch of
+// POL_BINS ints.
 __device__ void policy_features(const double* work, int nsub, double maxw, int j, int m, int n,
                                 float* feat, int* hist) {
-  const int lane = threadIdx.x & 63;
-  if (lane < POL_BINS) hist[lane] = 0;
-  wave_lds_sync();
-  for (int t = lane; t < nsub; t += 64) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (tid < POL_BINS) hist[tid] = 0;
+  __syncthreads();
+  for (int t = tid; t < nsub; t += nt) {
     const double r = maxw > 0.0 ? work[t] / maxw : 0.0;
     int b = (int)(r * (double)POL_BINS);
     b = b < 0 ? 0 : (b >= POL_BINS ? POL_BINS - 1 : b);
     atomicAdd(&hist[b], 1);
   }
-  wave_lds_sync();
-  if (lane < POL_BINS) feat[lane] = (float)hist[lane] / (float)nsub;
-  if (lane == POL_BINS) feat[lane] = (float)j / (float)(m < n ? m : n);
-  if (lane == POL_BINS + 1) {
+  __syncthreads();
+  if (tid < POL_BINS) feat[tid] = (float)hist[tid] / (float)nsub;
+  if (tid == POL_BINS) feat[tid] = (float)j / (float)(m < n ? m : n);
+  if (tid == POL_BINS + 1) {
     const double ar = (double)(m - j) / (double)(n - j);
-    feat[lane] = (float)((ar < 8.0 ? ar : 8.0) / 8.0);
+    feat[tid] = (float)((ar < 8.0 ? ar : 8.0) / 8.0);
   }
-  wave_lds_sync();
+  __syncthreads();
 }
 
 // Forward pass by one wave (lane = hidden unit); feat and hbuf (64 floats)
@@ -96,8 +97,11 @@ __device__ float2 policy_forward(const PolicyNet* __restrict__ P, const float* f
 // The step's proposal: features of work[j..n) (the top-k stage has left the
 // sorted candidates and kmax in st), the forward pass, and the candidate
 // count for the proposed tau; the greedy acceptance reads delta_step.
-__global__ __launch_bounds__(64) void k_policy(const double* __restrict__ work, int m, int n,
-                                               const PolicyNet* __restrict__ P, DevState* st) {
+constexpr int POL_THREADS = 1024;
+
+__global__ __launch_bounds__(POL_THREADS) void k_policy(const double* __restrict__ work, int m,
+                                                        int n, const PolicyNet* __restrict__ P,
+                                                        DevState* st) {
   __shared__ float feat[POL_IN];
   __shared__ float hbuf[POL_H];
   __shared__ int hist[POL_BINS];
@@ -106,6 +110,7 @@ __global__ __launch_bounds__(64) void k_policy(const double* __restrict__ work, 
   const int j = st->j, kmax = st->kmax;
   const double maxw = st->cand_val[0];
   policy_features(work + j, n - j, maxw, j, m, n, feat, hist);
+  if (threadIdx.x >= 64) return;
   const float2 y = policy_forward(P, feat, hbuf);
   double tau = P->tau_lo + (P->tau_hi - P->tau_lo) * (double)y.x;
   double delta = P->delta_lo + (P->delta_hi - P->delta_lo) * (double)y.y;
@@ -125,7 +130,7 @@ __global__ __launch_bounds__(64) void k_policy(const double* __restrict__ work, 
 
 void launch_policy(const double* work, int m, int n, const void* net, DevState* st,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_policy, dim3(1), dim3(64), 0, s, work, m, n,
+  hipLaunchKernelGGL(k_policy, dim3(1), dim3(POL_THREADS), 0, s, work, m, n,
                      static_cast<const PolicyNet*>(net), st);
 }
 

@@ -18,8 +18,9 @@ Families (seeded): geometric s_i = 10^(-d i/(k-1)), d ~ U[4, 14]; step,
 three clusters at 1, 1e-4, 1e-9 (sizes from a uniform split, values
 jittered by 10%); Kahan-type, A = U K D with K = diag(sin(t)^i)(I - cos(t) N)
 (N strictly upper triangular ones), t = 1.2, D = diag(1 + 0.01 N(0,1)).
-U, V have orthonormal columns from two passes of Cholesky QR of seeded
-Gaussians (batched on the device).
+U has orthonormal columns from two passes of Cholesky QR of a seeded
+Gaussian (m >= 2n here, so the Gaussian is well conditioned), V from the
+Householder QR of one (batched on the device).
 """
 import ctypes
 import os
@@ -89,8 +90,10 @@ def family_batch(torch, family, count, m, n, seed):
                                      dtype=torch.float64)
         a = torch.bmm(u, (kk[:k, :].unsqueeze(0) * d).contiguous())
         return a, None
-    v = _orthonormal(torch, torch.randn(count, n, k, generator=gen, device="cuda",
-                                        dtype=torch.float64))
+    # V is square when m >= n: Householder QR (a square Gaussian can be too
+    # ill-conditioned for Cholesky QR)
+    v = torch.linalg.qr(torch.randn(count, n, k, generator=gen, device="cuda",
+                                    dtype=torch.float64)).Q
     spectra = np.stack([_spectrum(family, k, rng) for _ in range(count)])
     s = torch.from_numpy(spectra).cuda().unsqueeze(1)
     a = torch.bmm((u * s).contiguous(), v.transpose(1, 2).contiguous())
@@ -217,3 +220,99 @@ def pack(torch, theta):
 def load_policy(torch, path=THETA_PATH):
     """The committed trained network, packed on the device."""
     return pack(torch, np.load(path, allow_pickle=False))
+
+
+def evaluate(torch, policy_net, m=8192, n=4096, seed=777, reps=3, thres=(0.9, 0.15), log=None):
+    """Default parameters against the learned proposal on one matrix of each
+    family: DM steps, time per factorisation, backward error and the
+    rank-revealing ratio |R_kk| / sigma_k (sigma by construction, or
+    computed for the Kahan-type matrix)."""
+    import qrdm_amd
+    out = {}
+    for fi, fam in enumerate(FAMILIES):
+        a, spectra = family_batch(torch, fam, 1, m, n, seed + fi)
+        a = a[0]
+        if spectra is None:
+            sv = torch.linalg.svdvals(a).cpu().numpy()
+        else:
+            sv = spectra[0]
+        res = {}
+        for label, net in (("default", None), ("learned", policy_net)):
+            ts = []
+            for _ in range(reps):
+                at = a.T.contiguous()
+                t = torch.zeros(min(m, n), dtype=torch.float64, device="cuda")
+                jp = np.zeros(n, np.int32)
+                nc = np.zeros(n, np.int32)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                info, _ = qrdm_amd.dgeqrdm_dev(at, m, t, jp, nc, np.array([thres[0], thres[1], 0.0]),
+                                               policy=net)
+                e1.record()
+                torch.cuda.synchronize()
+                if info != 0:
+                    raise RuntimeError("dgeqrdm_dev: %d" % info)
+                ts.append(e0.elapsed_time(e1))
+            steps = int(np.count_nonzero(nc))
+            R = torch.triu(at.T)
+            qr = R.T.contiguous()
+            lib().qrdm_dormqr_dev(0, m, n, min(m, n), ctypes.c_void_p(at.data_ptr()), m,
+                                  ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(qr.data_ptr()),
+                                  m, None)
+            perm = torch.from_numpy(jp.astype(np.int64) - 1).cuda()
+            be = (torch.linalg.norm(qr.T - a[:, perm]) / torch.linalg.norm(a)).item()
+            d = torch.abs(torch.diagonal(R)).cpu().numpy()
+            k = min(m, n)
+            ratio = d[:k] / sv[:k]
+            res[label] = dict(steps=steps, ms=float(np.median(ts)), backward=be,
+                              rr_min=float(ratio.min()), rr_max=float(ratio.max()),
+                              mean_block=float(min(m, n) / max(steps, 1)))
+        out[fam] = res
+        if log:
+            log("eval %s: %s" % (fam, res))
+    return out
+
+
+def main(argv=None):
+    """python -m qrdm_amd.learned train|eval [options] (on the GPU)."""
+    import argparse
+    import json
+    import time
+    import torch
+    ap = argparse.ArgumentParser(prog="python -m qrdm_amd.learned")
+    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("--matrices", type=int, default=20000, help="training matrices in all")
+    ap.add_argument("--m", type=int, default=2048)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=THETA_PATH)
+    ap.add_argument("--theta", default=THETA_PATH)
+    args = ap.parse_args(argv)
+
+    def log(s):
+        print(s, flush=True)
+
+    if args.cmd == "train":
+        t0 = time.time()
+        recs = collect_traces(torch, args.matrices // len(FAMILIES), m=args.m, n=args.n,
+                              seed=args.seed, log=log)
+        t1 = time.time()
+        theta, hist = train(torch, recs[:, :34], recs[:, 34:36], epochs=args.epochs,
+                            seed=args.seed, log=log)
+        t2 = time.time()
+        np.save(args.out, theta.astype(np.float32), allow_pickle=False)
+        log(json.dumps(dict(records=int(len(recs)), trace_s=round(t1 - t0, 1),
+                            train_s=round(t2 - t1, 1), final_loss=hist[-1][0],
+                            heldout_loss=hist[-1][1],
+                            default_block=float(recs[:, 36].mean()),
+                            label_block=float(recs[:, 37].mean()), out=args.out)))
+    else:
+        net = pack(torch, np.load(args.theta, allow_pickle=False))
+        log(json.dumps(evaluate(torch, net, log=log)))
+
+
+if __name__ == "__main__":
+    main()

@@ -233,3 +233,24 @@ def test_trace_records(gpu, oracle):
     assert (recs[:, 37] >= recs[:, 36]).all()
     say("trace: %d records, label block / default block = %.3f"
         % (len(recs), recs[:, 37].sum() / recs[:, 36].sum()))
+
+
+@pytest.mark.slow
+def test_trained_policy_against_defaults(gpu):
+    """The committed network (trained on the device on 20000 seeded 2048 x
+    1024 matrices of the three families, python -m qrdm_amd.learned train)
+    against the default parameters on one 8192 x 4096 matrix per family: no
+    loss in backward error or in the rank-revealing ratio min_k |R_kk| /
+    sigma_k (within a factor 2 of the defaults'), no more than one extra DM
+    step.  The gain (steps, time) is printed; it is measured, not asserted."""
+    from qrdm_amd import learned
+    torch = gpu
+    net = learned.load_policy(torch)
+    res = learned.evaluate(torch, net, reps=2, log=say)
+    for fam, r in res.items():
+        d, l = r["default"], r["learned"]
+        say("%s: steps %d -> %d, %.2f -> %.2f ms, min |R_kk|/sigma_k %.3g -> %.3g"
+            % (fam, d["steps"], l["steps"], d["ms"], l["ms"], d["rr_min"], l["rr_min"]))
+        assert d["backward"] <= 1e-13 and l["backward"] <= 1e-13
+        assert l["rr_min"] >= 0.5 * d["rr_min"]
+        assert l["steps"] <= d["steps"] + 1
