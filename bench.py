@@ -168,13 +168,14 @@ def backward_error(torch, qrdm_amd, work, pristine, tau, jp, stream):
     return (torch.linalg.norm(qr.T - A[:, perm]) / torch.linalg.norm(A)).item()
 
 
-def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, stream, ms_plain):
+def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, stream, ms_plain,
+                   which):
     """The plain FP64 path beside the FP32 and split-BF16x3 trailing updates
     on the same matrix: time per factorisation (same timing as the headline,
     no events) and the backward error of one factorisation of each."""
     m, n = args.m, args.n
     out = {}
-    for form in ("fp64", "fp32", "bf16x3"):
+    for form in ["fp64"] + list(which):
         ms_f = ms_plain
         if form != "fp64":
             timed(1, form)  # warm-up of the form's kernels
@@ -190,6 +191,74 @@ def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, s
     return out
 
 
+def batched_form(torch, qrdm_amd, steps, loop_count):
+    """4096 uneven matrices (m in [64, 1024], n in [32, min(m, 512)], spectra
+    log-uniform over [1e-10, 1]) in one persistent launch (autotuned launch
+    shape, replayed from a hipGraph), against a loop of the plain path."""
+    shapes = batch_shapes(4096, 4)
+    plan = qrdm_amd.BatchPlan(shapes, nb=64)
+    src, _ = batch_fill(torch, plan, 4)
+    q = plan.autotune(src)
+    work = torch.empty_like(src)
+    tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
+    jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
+    nc = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
+    info = torch.zeros(len(shapes), dtype=torch.int32, device="cuda")
+    total = 0.0
+    for rep in range(steps + 1):
+        work.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(work, tau, jp, nc, info, graph=True)
+        torch.cuda.synchronize()
+        if rep > 0:
+            total += time.perf_counter() - t0
+    ms = 1000.0 * total / steps
+    if int((info != 0).sum().item()) != 0:
+        raise RuntimeError("batched: non-zero info")
+    import numpy as np
+    work.copy_(src)
+    tau2 = torch.zeros_like(tau)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (m, n) in enumerate(shapes[:loop_count]):
+        o, t = int(plan.a_off[i]), int(plan.tau_off[i])
+        jv = np.zeros(n, np.int32)
+        nv = np.zeros(n, np.int32)
+        r, _ = qrdm_amd.dgeqrdm_dev(work[o:o + m * n].view(n, m), m, tau2[t:t + min(m, n)], jv,
+                                    nv, np.array([0.9, 0.15, 0.0]))
+        if r != 0:
+            raise RuntimeError("loop: info %d" % r)
+    torch.cuda.synchronize()
+    loop_ms = 1000.0 * (time.perf_counter() - t0)
+    fl = sum(flops(max(m, n), min(m, n)) for m, n in shapes)
+    plan.close()
+    return {"workload": "4096 uneven FP64 matrices, m in [64,1024], n in [32,min(m,512)], "
+                        "U diag(s) V^T with s log-uniform over [1e-10,1], nb=64",
+            "ms_per_batch": round(ms, 2), "tfs": round(fl / (ms / 1000.0) / 1e12, 4),
+            "flops": fl, "per_matrix_loop_matrices": min(loop_count, len(shapes)),
+            "per_matrix_loop_ms": round(loop_ms, 1),
+            "speedup_vs_loop": (round(loop_ms / ms, 2) if loop_count >= len(shapes) else None),
+            "launch": q}
+
+
+def learned_form(torch):
+    """The committed (tau, delta) network against the defaults on one
+    8192 x 4096 matrix of each training family."""
+    from qrdm_amd import learned
+    net = learned.load_policy(torch)
+    res = learned.evaluate(torch, net, reps=2)
+    out = {}
+    for fam, r in res.items():
+        d, l = r["default"], r["learned"]
+        out[fam] = {"steps": [d["steps"], l["steps"]], "ms": [round(d["ms"], 3), round(l["ms"], 3)],
+                    "backward": [float("%.3g" % d["backward"]), float("%.3g" % l["backward"])],
+                    "min_Rkk_over_sigma": [float("%.3g" % d["rr_min"]),
+                                           float("%.3g" % l["rr_min"])]}
+    return {"workload": "8192x4096 FP64, one matrix per family (geometric, step, Kahan-type); "
+                        "pairs are [defaults, learned]", "families": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +269,11 @@ def main():
     ap.add_argument("--rank", type=int, default=2048)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forms", default="fp32,bf16x3,batched,learned",
+                    help="comma list of the added forms to measure beside the plain path "
+                         "(fp32, bf16x3, batched, learned; empty for none)")
+    ap.add_argument("--batch-loop", type=int, default=4096,
+                    help="matrices in the per-matrix loop the batched form is compared with")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -308,8 +382,17 @@ def main():
                        "dm_steps": nsteps_dm,
                        "kernel_classes_one_factorisation": others},
             "roofline": roofline, "roofline_apply": roofline_apply}
+    forms = [f for f in args.forms.split(",") if f]
     line["forms"] = trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp,
-                                   stream, ms)
+                                   stream, ms, [f for f in forms if f in ("fp32", "bf16x3")])
+    for key, fn in (("batched", lambda: batched_form(torch, qrdm_amd, 3, args.batch_loop)),
+                    ("learned", lambda: learned_form(torch))):
+        if key not in forms:
+            continue
+        try:
+            line["forms"][key] = fn()
+        except Exception as e:  # reported in the line, the headline stands
+            line["forms"][key] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
