@@ -103,6 +103,11 @@ def cpu_baseline(n_sample=8192, rank_sample=2048):
             "seconds": round(t_dm, 3), "dgeqp3_seconds": round(t_qp3, 3)}
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def batch_shapes(count, seed, mmin=64, mmax=1024, nmin=32, nmax=512):
     """Uneven batch shapes: m uniform in [mmin, mmax], n uniform in
     [nmin, min(m, nmax)]."""
@@ -269,9 +274,12 @@ def main():
     ap.add_argument("--rank", type=int, default=2048)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3,batched,learned",
+    ap.add_argument("--forms", default="fp32,bf16x3",
                     help="comma list of the added forms to measure beside the plain path "
-                         "(fp32, bf16x3, batched, learned; empty for none)")
+                         "(fp32, bf16x3, batched, learned; empty for none).  batched and "
+                         "learned are left out by default: a run with both in one process "
+                         "ended in a host-heap abort whose cause is not identified; they are "
+                         "measured by tests/test_gpu_batched.py and python -m qrdm_amd.learned")
     ap.add_argument("--batch-loop", type=int, default=4096,
                     help="matrices in the per-matrix loop the batched form is compared with")
     args = ap.parse_args()
@@ -324,6 +332,7 @@ def main():
     # timed steps, no events on the stream
     qrdm_amd.profile(enable=False, reset=True)
     ms, last_nc = timed(args.steps)
+    progress("timed steps: %.3f ms per factorisation" % ms)
     nsteps_dm = int(np.count_nonzero(last_nc))
     tfs = flops(m, n) / (ms / 1000.0) / 1e12
     # the same steps again with HIP events around every k_gemm1 launch (each
@@ -382,6 +391,7 @@ def main():
                        "dm_steps": nsteps_dm,
                        "kernel_classes_one_factorisation": others},
             "roofline": roofline, "roofline_apply": roofline_apply}
+    progress("profiled steps done")
     forms = [f for f in args.forms.split(",") if f]
     line["forms"] = trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp,
                                    stream, ms, [f for f in forms if f in ("fp32", "bf16x3")])
@@ -393,8 +403,11 @@ def main():
             line["forms"][key] = fn()
         except Exception as e:  # reported in the line, the headline stands
             line["forms"][key] = {"error": "%s: %s" % (type(e).__name__, e)}
+        progress("form %s done" % key)
+    progress("forms done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
+        progress("cpu baseline done")
     print(json.dumps(line), flush=True)
 
 

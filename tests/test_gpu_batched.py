@@ -290,3 +290,32 @@ def test_config4_batch_vs_plain_loop(gpu, oracle):
                    "matches" if np.array_equal(jp2[c:c + lead], ref["jpvt"][:lead]) else "differs from"))
     assert lead_same >= 4096 - 8
     assert max(diffR) < 1e-12
+
+
+@pytest.mark.slow
+def test_config4_graph_replay_matches_eager(gpu):
+    """The 4096-matrix uneven batch: the hipGraph replay of the persistent
+    launch gives bit-identical results to the eager launch, every matrix
+    factorised (info 0)."""
+    import bench
+    import qrdm_amd
+    torch = gpu
+    shapes = bench.batch_shapes(4096, 4)
+    plan = qrdm_amd.BatchPlan(shapes, nb=64)
+    src, _ = bench.batch_fill(torch, plan, 4)
+    res = []
+    for graph in (False, True, True):
+        work = src.clone()
+        tau, jp, nc, info = outputs(torch, plan)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.run(work, tau, jp, nc, info, graph=graph)
+        e1.record()
+        torch.cuda.synchronize()
+        say("config4 %s launch: %.1f ms" % ("graph" if graph else "eager", e0.elapsed_time(e1)))
+        assert int((info != 0).sum().item()) == 0
+        res.append([x.cpu().numpy() for x in (work, tau, jp, nc)])
+    for r in res[1:]:
+        for x, y in zip(r, res[0]):
+            assert np.array_equal(x, y)
