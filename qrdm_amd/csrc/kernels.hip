@@ -1389,6 +1389,89 @@ __global__ __launch_bounds__(256) void k_gemm2(double* A, int lda, int j, int m,
   }
 }
 
+// k_gemm2 with 32-byte row vectors (lda % 4 == 0, 32-byte aligned A): the
+// tile starts at the 4-row boundary at or below j, and lane li of a lane
+// group holds rows 4li..4li+3 of its columns (MFMA tile t = row 4li + t), so
+// each column of C moves as one dbl4 per lane (16 lanes: 512 contiguous
+// bytes).  Rows above j (V is zero there, so they would be rewritten with
+// their own value) and rows past m are not stored.  (Sweeping 2, 4 or 8
+// column tiles per workgroup with the V tile kept in LDS was measured
+// slower: fewer workgroups, one wave per SIMD.)
+__global__ __launch_bounds__(256) void k_gemm2_vec(double* A, int lda, int j, int m, int c0,
+                                                   int ncol, const double* __restrict__ V,
+                                                   int ldv, const double* __restrict__ W,
+                                                   int ldw, const DevState* st) {
+  __shared__ double Vs[64][KMAX + 1];
+  if (st) {
+    if (dm_stopped(st)) return;
+    j = st->j;
+    c0 = st->c0;
+    ncol = ldw - c0;
+  }
+  const int r0 = j & ~3;
+  if (r0 + (int)blockIdx.x * 64 >= m || (int)blockIdx.y * 64 >= ncol) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = lane >> 4, li = lane & 15;
+  const int rbase = r0 + blockIdx.x * 64;
+  const int cbase = blockIdx.y * 64 + wave * 16;
+  // the lane's rows rbase + 4li .. +3 (a row group starting past m reads the
+  // last group inside the column, never stored)
+  const int rclamp = ((m + 3) & ~3) - 4;
+  const int rl = rbase + 4 * li;
+  const int rld = rl < rclamp ? rl : rclamp;
+  dbl4 cval[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int cc = cbase + g + 4 * v;
+    const double* col = A + (size_t)(c0 + (cc < ncol ? cc : 0)) * lda;
+    cval[v] = *reinterpret_cast<const dbl4*>(col + rld);
+  }
+  {
+    double vv[64 * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < 64 * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      const int k = e >> 6, r = e & 63;
+      vv[q] = (rbase + r < m) ? V[(size_t)k * ldv + rbase + r] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 64 * KMAX / 256; q++) {
+      const int e = tid + 256 * q;
+      Vs[e & 63][e >> 6] = vv[q];
+    }
+  }
+  __syncthreads();
+  dbl4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  const int wc = cbase + li;
+  const bool wok = wc < ncol;
+#pragma unroll 4
+  for (int k0 = 0; k0 < KMAX; k0 += 4) {
+    const double a = wok ? W[(size_t)(k0 + g) * ldw + wc] : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Vs[4 * li + t][k0 + g], acc[t], 0, 0, 0);
+  }
+  // acc[t][v]: column cbase + g + 4v, row rbase + 4li + t
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const int cc = cbase + g + 4 * v;
+    if (cc >= ncol) continue;
+    double* col = A + (size_t)(c0 + cc) * lda;
+    dbl4 o;
+#pragma unroll
+    for (int t = 0; t < 4; t++) o[t] = cval[v][t] - acc[t][v];
+    if (rl >= j && rl + 3 < m) {
+      *reinterpret_cast<dbl4*>(col + rl) = o;
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (rl + t >= j && rl + t < m) col[rl + t] = o[t];
+    }
+  }
+}
+
 // Row slabs of k_gemm1's split reduction for columns [c0, n) and rows
 // [j & ~(G1_CHUNK - 1), m): about one workgroup per CU.
 struct G1Geom {
@@ -1451,8 +1534,13 @@ void launch_larfb_apply(double* A, int lda, int j, int m, int c0, int n, const d
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const int ctiles = (ncol + 63) / 64;
-  hipLaunchKernelGGL(k_gemm2, dim3((mrow + 63) / 64, ctiles), dim3(256), 0, s, A, lda, j, m, c0,
-                     ncol, V, ldv, W, n, st);
+  const bool vec = (lda % 4 == 0) && m >= 4 && ((reinterpret_cast<uintptr_t>(A) & 31) == 0);
+  if (vec)  // the tile starts at j & ~3: one more row tile at most
+    hipLaunchKernelGGL(k_gemm2_vec, dim3((m - (j & ~3) + 63) / 64, ctiles), dim3(256), 0, s, A,
+                       lda, j, m, c0, ncol, V, ldv, W, n, st);
+  else
+    hipLaunchKernelGGL(k_gemm2, dim3((mrow + 63) / 64, ctiles), dim3(256), 0, s, A, lda, j, m,
+                       c0, ncol, V, ldv, W, n, st);
 }
 
 void launch_larfb(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
