@@ -1109,11 +1109,13 @@ constexpr int G1_CHUNK = 32;
 constexpr int G1_MAX_SPLIT = 16;  // row slabs of the split reduction (Wp holds this many)
 
 // k_gemm1: the partial products Wp[y] = V^T C over row slab y.  A workgroup
-// covers 256 columns of C: wave w owns the 64-column tile 4*blockIdx.x + w and
-// the four waves walk the same 16-row chunks of the slab, so the V chunk
-// (16 rows x 64 columns) is loaded once per workgroup into LDS (double
-// buffered, one barrier per chunk) while each wave streams its own C tile
-// through registers one chunk ahead of its MFMAs.  Lane group g = lane/16
+// covers 64 * G1_NWG columns of C: wave w owns the 64-column tile
+// G1_NWG*blockIdx.x + w and the waves walk the same 16-row chunks of the slab,
+// so the V chunk (16 rows x 64 columns) is loaded once per workgroup into LDS
+// (double buffered, one barrier per chunk) while each wave streams its own C
+// tile through registers one chunk ahead of its MFMAs.  Two-wave workgroups,
+// about two per CU, measured faster than four-wave ones, one per CU, at the
+// same number of row slabs (k_gemm1 0.44 -> 0.49 of FP64 MFMA peak at 8192^2).  Lane group g = lane/16
 // holds rows 4g..4g+3 of a chunk (MFMA k-step q is row 4g + q).  Only the
 // slab's last chunk can run past m; its rows >= re are masked.  Each wave
 // writes its 64 x 64 partial tile straight from its accumulators.
@@ -1121,9 +1123,10 @@ constexpr int G1_MAX_SPLIT = 16;  // row slabs of the split reduction (Wp holds 
 // sized for the lowest j, so whole workgroups past ncol return and row slabs
 // past m write zero partials.
 constexpr int G1_VPAD = 4;  // LDS row padding of the V chunk (keeps 32-byte alignment)
+constexpr int G1_NWG = 2;   // waves (64-column tiles) per workgroup
 
 template <bool VEC>
-__global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int lda, int j,
+__global__ __launch_bounds__(64 * G1_NWG) void k_gemm1(const double* __restrict__ A, int lda, int j,
                                                int m, int c0, int ncol,
                                                const double* __restrict__ V, int ldv,
                                                int slab, double* Wp, int ldw,
@@ -1135,13 +1138,13 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
     c0 = st->c0;
     ncol = ldw - c0;
   }
-  if ((int)blockIdx.x * 256 >= ncol) return;
+  if ((int)blockIdx.x * 64 * G1_NWG >= ncol) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int base = j & ~(G1_CHUNK - 1);
   const int rs = base + blockIdx.y * slab;
   const int re = max(min(rs + slab, m), rs);  // a slab past m is empty
-  const int cbw = blockIdx.x * 256 + wave * 64;  // this wave's tile (relative to c0)
+  const int cbw = blockIdx.x * 64 * G1_NWG + wave * 64;  // this wave's tile (relative to c0)
   // columns past ncol read a valid column instead; they are never stored
   const double* Cc[4];
 #pragma unroll
@@ -1159,10 +1162,12 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
   // last 4-row group that starts inside the column: with lda % 4 == 0 (the
   // vector path's requirement) rows up to round_up(m, 4) - 1 exist in memory
   const int rclamp = ((m + 3) & ~3) - 4;
-  // V chunk loader: thread tid brings rows 4*(tid&3).. of V column tid>>2
-  const int vcol = tid >> 2, vq = 4 * (tid & 3);
-  const double* Vcol = V + (size_t)vcol * ldv;
-  auto load_v = [&](int c) -> dbl4 {
+  // V chunk loader: thread tid brings rows 4*(tid&3).. of the V columns
+  // (tid>>2) + 16 * G1_NWG * h, h < VPT
+  constexpr int VPT = 4 / G1_NWG;
+  const int vq = 4 * (tid & 3);
+  auto load_v = [&](int c, int h) -> dbl4 {
+    const double* Vcol = V + (size_t)((tid >> 2) + 16 * G1_NWG * h) * ldv;
     const int r = rs + CH * c + vq;
     dbl4 v;
     if constexpr (VEC) {
@@ -1174,11 +1179,11 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
     return v;
   };
   // rows >= re (past m, where V may hold anything) are stored as zero
-  auto store_v = [&](int buf, int c, dbl4 v) {
+  auto store_v = [&](int buf, int c, int h, dbl4 v) {
     const int r = rs + CH * c + vq;
 #pragma unroll
     for (int q = 0; q < 4; q++) v[q] = (r + q < re) ? v[q] : 0.0;
-    *reinterpret_cast<dbl4*>(&Vs[buf][vcol][vq]) = v;
+    *reinterpret_cast<dbl4*>(&Vs[buf][(tid >> 2) + 16 * G1_NWG * h][vq]) = v;
   };
   auto load_c = [&](int c, dbl4 (&cc)[4]) {
     const int r = rs + CH * c + 4 * g;
@@ -1197,7 +1202,8 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
   };
   dbl4 ca[4];
   if (nch > 0) {
-    store_v(0, 0, load_v(0));
+#pragma unroll
+    for (int h = 0; h < VPT; h++) store_v(0, 0, h, load_v(0, h));
     load_c(0, ca);
   }
   __syncthreads();
@@ -1206,7 +1212,9 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
     const int buf = c & 1;
     // chunk c + 1 (clamped to the last chunk, then unused) in flight
     const int cn_idx = min(c + 1, nch - 1);
-    const dbl4 vnext = load_v(cn_idx);
+    dbl4 vnext[VPT];
+#pragma unroll
+    for (int h = 0; h < VPT; h++) vnext[h] = load_v(cn_idx, h);
     dbl4 cn[4];
     load_c(cn_idx, cn);
     dbl4 vc[4];
@@ -1230,7 +1238,8 @@ __global__ __launch_bounds__(256) void k_gemm1(const double* __restrict__ A, int
         for (int u = 0; u < 4; u++)
           acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(vc[t][q], ca[u][q], acc[t][u], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    store_v(buf ^ 1, cn_idx, vnext);
+#pragma unroll
+    for (int h = 0; h < VPT; h++) store_v(buf ^ 1, cn_idx, h, vnext[h]);
 #pragma unroll
     for (int u = 0; u < 4; u++) ca[u] = cn[u];
     __syncthreads();
@@ -1473,7 +1482,8 @@ __global__ __launch_bounds__(256) void k_gemm2_vec(double* A, int lda, int j, in
 }
 
 // Row slabs of k_gemm1's split reduction for columns [c0, n) and rows
-// [j & ~(G1_CHUNK - 1), m): about one workgroup per CU.
+// [j & ~(G1_CHUNK - 1), m): about 256 * 4 / G1_NWG workgroups (two per CU),
+// so as many slabs as one 256-column workgroup per CU would have.
 struct G1Geom {
   int cgroups, ns, slab;
 };
@@ -1481,8 +1491,8 @@ struct G1Geom {
 static G1Geom gemm1_geom(int j, int m, int c0, int n, int nsplit) {
   const int ncol = n - c0;
   G1Geom g;
-  g.cgroups = (ncol + 255) / 256;
-  int want = (256 + g.cgroups - 1) / g.cgroups;
+  g.cgroups = (ncol + 64 * G1_NWG - 1) / (64 * G1_NWG);
+  int want = (1024 / G1_NWG + g.cgroups - 1) / g.cgroups;
   if (want > nsplit) want = nsplit;
   if (want > G1_MAX_SPLIT) want = G1_MAX_SPLIT;
   const int base = j & ~(G1_CHUNK - 1);
@@ -1506,10 +1516,10 @@ void launch_gemm1(double* A, int lda, int j, int m, int c0, int n, const double*
                       ((reinterpret_cast<uintptr_t>(A) & 31) == 0) &&
                       ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
   if (aligned)
-    hipLaunchKernelGGL(k_gemm1<true>, dim3(g.cgroups, g.ns), dim3(256), 0, s, A, lda, j, m, c0,
+    hipLaunchKernelGGL(k_gemm1<true>, dim3(g.cgroups, g.ns), dim3(64 * G1_NWG), 0, s, A, lda, j, m, c0,
                        ncol, V, ldv, g.slab, Wp, n, st);
   else
-    hipLaunchKernelGGL(k_gemm1<false>, dim3(g.cgroups, g.ns), dim3(256), 0, s, A, lda, j, m, c0,
+    hipLaunchKernelGGL(k_gemm1<false>, dim3(g.cgroups, g.ns), dim3(64 * G1_NWG), 0, s, A, lda, j, m, c0,
                        ncol, V, ldv, g.slab, Wp, n, st);
 }
 

@@ -37,12 +37,13 @@ __device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16
 constexpr int MX_CH = 32;  // rows per chunk
 
 // Partial products Wp[y] = V^T C over row slab y (k_gemm1's split), FP32
-// accumulation.  Workgroup: 256 columns, wave w the 64-column tile
-// 4*blockIdx.x + w; lane group g = lane/16 holds rows 8g..8g+7 of a 32-row
-// chunk.  The V chunk is converted once per workgroup into LDS (double
-// buffered); each wave streams its C tile through registers one chunk ahead.
+// accumulation.  Workgroup: 64 * G1_NWG columns (as k_gemm1), wave w the
+// 64-column tile G1_NWG*blockIdx.x + w; lane group g = lane/16 holds rows
+// 8g..8g+7 of a 32-row chunk.  The V chunk is converted once per workgroup
+// into LDS (double buffered); each wave streams its C tile through registers
+// one chunk ahead.
 template <int PREC, bool VEC>
-__global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, int lda, int j,
+__global__ __launch_bounds__(64 * G1_NWG) void k_gemm1_mx(const double* __restrict__ A, int lda, int j,
                                                   int m, int c0, int ncol,
                                                   const double* __restrict__ V, int ldv,
                                                   int slab, double* Wp, int ldw,
@@ -57,13 +58,13 @@ __global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, 
     c0 = st->c0;
     ncol = ldw - c0;
   }
-  if ((int)blockIdx.x * 256 >= ncol) return;
+  if ((int)blockIdx.x * 64 * G1_NWG >= ncol) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int base = j & ~(G1_CHUNK - 1);
   const int rs = base + blockIdx.y * slab;
   const int re = max(min(rs + slab, m), rs);
-  const int cbw = blockIdx.x * 256 + wave * 64;
+  const int cbw = blockIdx.x * 64 * G1_NWG + wave * 64;
   // columns past ncol read a valid column instead; they are never stored
   const double* Cc[4];
 #pragma unroll
@@ -93,11 +94,14 @@ __global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, 
       for (int q = 0; q < 8; q++) x[q] = col[min(r + q, m - 1)];
     }
   };
-  // V chunk loader: thread tid brings rows 8*(tid&3).. of V column tid>>2;
-  // rows >= re (past m, where V may hold anything) are stored as zero
-  const int vcol = tid >> 2, vq = 8 * (tid & 3);
-  const double* Vcol = V + (size_t)vcol * ldv;
-  auto store_v = [&](int buf, int c, const double (&x)[8]) {
+  // V chunk loader: thread tid brings rows 8*(tid&3).. of the V columns
+  // (tid>>2) + 16 * G1_NWG * h, h < VPT; rows >= re (past m, where V may hold
+  // anything) are stored as zero
+  constexpr int VPT = 4 / G1_NWG;
+  const int vq = 8 * (tid & 3);
+  auto vcolp = [&](int h) { return V + (size_t)((tid >> 2) + 16 * G1_NWG * h) * ldv; };
+  auto store_v = [&](int buf, int c, int h, const double (&x)[8]) {
+    const int vcol = (tid >> 2) + 16 * G1_NWG * h;
     const int r = rs + MX_CH * c + vq;
     float f[8];
 #pragma unroll
@@ -114,9 +118,12 @@ __global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, 
   };
   double ca[4][8];
   if (nch > 0) {
-    double x[8];
-    load8(Vcol, rs + vq, x);
-    store_v(0, 0, x);
+#pragma unroll
+    for (int h = 0; h < VPT; h++) {
+      double x[8];
+      load8(vcolp(h), rs + vq, x);
+      store_v(0, 0, h, x);
+    }
 #pragma unroll
     for (int u = 0; u < 4; u++) load8(Cc[u], rs + 8 * g, ca[u]);
   }
@@ -125,8 +132,9 @@ __global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, 
   for (int c = 0; c < nch; c++) {
     const int buf = c & 1;
     const int cn_idx = min(c + 1, nch - 1);  // chunk c + 1 (clamped, then unused) in flight
-    double vnext[8];
-    load8(Vcol, rs + MX_CH * cn_idx + vq, vnext);
+    double vnext[VPT][8];
+#pragma unroll
+    for (int h = 0; h < VPT; h++) load8(vcolp(h), rs + MX_CH * cn_idx + vq, vnext[h]);
     double cn[4][8];
 #pragma unroll
     for (int u = 0; u < 4; u++) load8(Cc[u], rs + MX_CH * cn_idx + 8 * g, cn[u]);
@@ -191,7 +199,8 @@ __global__ __launch_bounds__(256) void k_gemm1_mx(const double* __restrict__ A, 
         }
       __builtin_amdgcn_sched_barrier(0);
     }
-    store_v(buf ^ 1, cn_idx, vnext);
+#pragma unroll
+    for (int h = 0; h < VPT; h++) store_v(buf ^ 1, cn_idx, h, vnext[h]);
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -334,13 +343,13 @@ void launch_gemm1_prec(int prec, double* A, int lda, int j, int m, int c0, int n
   }
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
-  const G1Geom g = gemm1_geom(j, m, c0, n, nsplit);
+  const G1Geom g = gemm1_geom(j, m, c0, n, nsplit);  // k_gemm1's geometry
   const bool aligned = (lda % 4 == 0) && (ldv % 4 == 0) && m >= 4 &&
                        ((reinterpret_cast<uintptr_t>(A) & 31) == 0) &&
                        ((reinterpret_cast<uintptr_t>(V) & 31) == 0);
   const dim3 grid(g.cgroups, g.ns);
 #define QRDM_G1MX(P, VEC)                                                                     \
-  hipLaunchKernelGGL((k_gemm1_mx<P, VEC>), grid, dim3(256), 0, s, A, lda, j, m, c0, ncol, V, \
+  hipLaunchKernelGGL((k_gemm1_mx<P, VEC>), grid, dim3(64 * G1_NWG), 0, s, A, lda, j, m, c0, ncol, V, \
                      ldv, g.slab, Wp, n, st)
   if (prec == TRAIL_FP32) {
     if (aligned) QRDM_G1MX(TRAIL_FP32, true);
