@@ -1260,7 +1260,11 @@ __global__ __launch_bounds__(64 * G1_NWG) void k_gemm1(const double* __restrict_
 
 // k_wsum: W = op(T) sum_y Wp[y] for one 64-column tile per workgroup; the
 // partials are summed in slab order (deterministic), op(T) applied on the
-// matrix cores (T upper triangular, zero below).  trans = 1: T^T.
+// matrix cores (T upper triangular, zero below).  trans = 1: T^T.  NSMAX:
+// 8 or 16, the smaller one not below the slab count.  (Up to 32 slabs made
+// k_gemm1 faster on narrow steps, 0.49 -> 0.51 of peak, but the longer sums
+// cost more than that.)
+template <int NSMAX>
 __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wp, int ns, int ldw,
                                               int c0, int ncol, const double* __restrict__ T,
                                               int trans, double* W, const DevState* st) {
@@ -1289,13 +1293,15 @@ __global__ __launch_bounds__(256) void k_wsum(const double* __restrict__ Wp, int
     const int e = tid + 256 * q;
     const int i = e >> 6, cc = cb + (e & 63);
     const int ccl = cc < ncol ? cc : ncol - 1;
-    double pv[G1_MAX_SPLIT];
+    // all NSMAX loads in flight (slabs past ns re-read the last one and
+    // are not added)
+    double pv[NSMAX];
 #pragma unroll
-    for (int y = 0; y < G1_MAX_SPLIT; y++)
+    for (int y = 0; y < NSMAX; y++)
       pv[y] = Wp[((size_t)(y < ns ? y : ns - 1) * KMAX + i) * ldw + ccl];
     double sum = 0.0;
 #pragma unroll
-    for (int y = 0; y < G1_MAX_SPLIT; y++) sum += (y < ns) ? pv[y] : 0.0;
+    for (int y = 0; y < NSMAX; y++) sum += (y < ns) ? pv[y] : 0.0;
     Ws[i][e & 63] = sum;
   }
   __syncthreads();
@@ -1528,8 +1534,12 @@ void launch_wsum(int j, int m, int c0, int n, const double* T, int trans, const 
   const int ncol = n - c0, mrow = m - j;
   if (ncol <= 0 || mrow <= 0) return;
   const G1Geom g = gemm1_geom(j, m, c0, n, nsplit);
-  hipLaunchKernelGGL(k_wsum, dim3((ncol + 63) / 64), dim3(256), 0, s, Wp, g.ns, n, c0, ncol, T,
-                     trans, W, st);
+  const dim3 grid((ncol + 63) / 64);
+  static_assert(G1_MAX_SPLIT == 16, "k_wsum instances cover 16 slabs");
+  if (g.ns <= 8)
+    hipLaunchKernelGGL(k_wsum<8>, grid, dim3(256), 0, s, Wp, g.ns, n, c0, ncol, T, trans, W, st);
+  else
+    hipLaunchKernelGGL(k_wsum<16>, grid, dim3(256), 0, s, Wp, g.ns, n, c0, ncol, T, trans, W, st);
 }
 
 void launch_larfb_w(double* A, int lda, int j, int m, int c0, int n, const double* V, int ldv,
