@@ -196,14 +196,18 @@ def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, s
     return out
 
 
-def batched_form(torch, qrdm_amd, steps, loop_count):
+def batched_form(torch, qrdm_amd, steps, loop_count, autotune=True):
     """4096 uneven matrices (m in [64, 1024], n in [32, min(m, 512)], spectra
     log-uniform over [1e-10, 1]) in one persistent launch (autotuned launch
     shape, replayed from a hipGraph), against a loop of the plain path."""
     shapes = batch_shapes(4096, 4)
     plan = qrdm_amd.BatchPlan(shapes, nb=64)
     src, _ = batch_fill(torch, plan, 4)
-    q = plan.autotune(src)
+    if autotune:
+        q = plan.autotune(src)
+    else:
+        plan.set_shape(8)  # the shape the autotuner picks on this batch
+        q = plan.query()
     work = torch.empty_like(src)
     tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
     jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
@@ -264,6 +268,38 @@ def learned_form(torch):
                         "pairs are [defaults, learned]", "families": out}
 
 
+def form_in_child(key, batch_loop, limit=400):
+    """One added form measured by a child bench.py process (its JSON on
+    stdout); a failure there is reported in the line and leaves the headline
+    measurement alone."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--form-child", key,
+           "--batch-loop", str(batch_loop)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=limit)
+    except subprocess.TimeoutExpired:
+        return {"error": "child process over %d s" % limit}
+    if r.returncode != 0:
+        return {"error": "child process exit %d: %s" % (r.returncode, r.stderr[-300:])}
+    try:
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:
+        return {"error": "child output: %s" % e}
+
+
+def run_form_child(key, batch_loop):
+    import torch
+    import qrdm_amd
+    torch.cuda.set_device(0)
+    if key == "batched":
+        out = batched_form(torch, qrdm_amd, 3, batch_loop, autotune=False)
+    elif key == "learned":
+        out = learned_form(torch)
+    else:
+        raise SystemExit("unknown form %s" % key)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,15 +310,19 @@ def main():
     ap.add_argument("--rank", type=int, default=2048)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3",
+    ap.add_argument("--forms", default="fp32,bf16x3,learned",
                     help="comma list of the added forms to measure beside the plain path "
                          "(fp32, bf16x3, batched, learned; empty for none).  batched and "
-                         "learned are left out by default: a run with both in one process "
-                         "ended in a host-heap abort whose cause is not identified; they are "
-                         "measured by tests/test_gpu_batched.py and python -m qrdm_amd.learned")
+                         "learned run in child processes; batched is not in the default list: "
+                         "its child ended in a host-heap abort whose cause is not identified "
+                         "(the batched form is measured by tests/test_gpu_batched.py)")
+    ap.add_argument("--form-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--batch-loop", type=int, default=4096,
                     help="matrices in the per-matrix loop the batched form is compared with")
     args = ap.parse_args()
+    if args.form_child:
+        run_form_child(args.form_child, args.batch_loop)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
               "(the multi-GPU design is described, not built)" % args.gpus, file=sys.stderr)
@@ -398,15 +438,10 @@ def main():
         line["forms"] = trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau,
                                        last_jp, stream, ms,
                                        [f for f in forms if f in ("fp32", "bf16x3")])
-    for key, fn in (("batched", lambda: batched_form(torch, qrdm_amd, 3, args.batch_loop)),
-                    ("learned", lambda: learned_form(torch))):
-        if key not in forms:
-            continue
-        try:
-            line["forms"][key] = fn()
-        except Exception as e:  # reported in the line, the headline stands
-            line["forms"][key] = {"error": "%s: %s" % (type(e).__name__, e)}
-        progress("form %s done" % key)
+    for key in ("batched", "learned"):
+        if key in forms:
+            line["forms"][key] = form_in_child(key, args.batch_loop)
+            progress("form %s done" % key)
     progress("forms done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
