@@ -198,16 +198,13 @@ def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, s
 
 def batched_form(torch, qrdm_amd, steps, loop_count, autotune=True):
     """4096 uneven matrices (m in [64, 1024], n in [32, min(m, 512)], spectra
-    log-uniform over [1e-10, 1]) in one persistent launch (autotuned launch
-    shape, replayed from a hipGraph), against a loop of the plain path."""
+    log-uniform over [1e-10, 1]) in one persistent launch (4 waves per
+    workgroup, nb 64, replayed from a hipGraph), against a loop of the plain
+    path."""
     shapes = batch_shapes(4096, 4)
     plan = qrdm_amd.BatchPlan(shapes, nb=64)
     src, _ = batch_fill(torch, plan, 4)
-    if autotune:
-        q = plan.autotune(src)
-    else:
-        plan.set_shape(8)  # the shape the autotuner picks on this batch
-        q = plan.query()
+    q = plan.autotune(src) if autotune else plan.query()
     work = torch.empty_like(src)
     tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
     jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")

@@ -167,11 +167,13 @@ int qrdm_batch_plan_set_policy(qrdm_batch_plan *plan, const void *d_policy);
 int qrdm_batch_trace(qrdm_batch_plan *plan, double *d_a, double *d_tau, int *d_jpvt,
                      int *d_ncols, int *d_info, const double *thres, float *d_records, int cap,
                      unsigned *d_count, const float *ranges, void *stream);
-/* Sets the launch shape by hand (waves per workgroup, 4 or 8). */
+/* Sets the launch shape by hand: waves per workgroup, 4 (the one shape
+ * built; 8 is switched off and returns -1). */
 int qrdm_batch_plan_set_shape(qrdm_batch_plan *plan, int waves);
-/* Times the launch shapes (4 or 8 waves per workgroup; and nb when the plan
- * was created with nb <= 0) on a scratch copy of the batch d_a (left
- * unchanged) and keeps the fastest.  thres = {delta, tau}.  Synchronous. */
+/* Times the candidate launches (nb in {16, 32, 64} when the plan was
+ * created with nb <= 0, else the plan's nb; 4 waves per workgroup) on a
+ * scratch copy of the batch d_a (left unchanged) and keeps the fastest.
+ * thres = {delta, tau}.  Synchronous. */
 int qrdm_batch_autotune(qrdm_batch_plan *plan, const double *d_a, const double *thres,
                         void *stream);
 /* Factorises the whole batch in place: one kernel launch on stream,

@@ -205,12 +205,14 @@ class BatchPlan:
         return dict(waves=int(out[0]), grid=int(out[1]), nb=int(out[2]), tuned=int(out[3]))
 
     def set_shape(self, waves):
-        """Launch shape by hand: 4 or 8 waves per workgroup."""
+        """Launch shape by hand: 4 waves per workgroup (the one shape built;
+        8 is switched off)."""
         if lib().qrdm_batch_plan_set_shape(self._p, waves) != 0:
-            raise ValueError("waves must be 4 or 8")
+            raise ValueError(last_error())
 
     def autotune(self, a, thres=(0.9, 0.15), stream=None):
-        """Times the launch shapes on a scratch copy of the batch tensor a."""
+        """Times the candidate launches (nb in {16, 32, 64} for a plan made
+        with nb <= 0) on a scratch copy of the batch tensor a."""
         import numpy as np
         th = np.ascontiguousarray(list(thres) + [0.0] * (3 - len(thres)), dtype=np.float64)
         info = lib().qrdm_batch_autotune(self._p, ctypes.c_void_p(a.data_ptr()),

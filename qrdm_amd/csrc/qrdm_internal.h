@@ -201,7 +201,8 @@ void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n
 // nullptr; trace: per-step training records, or nullptr (ranges: the label
 // normalisation tau_lo, tau_hi, delta_lo, delta_hi).
 constexpr int BATCH_NMAX = 1024;
-void launch_dm_batch(int waves, int grid, const void* items, const int* sched,
+// Returns -1 (nothing launched) for a launch shape other than 4 waves.
+int launch_dm_batch(int waves, int grid, const void* items, const int* sched,
                      const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
                      double tol3z, const void* policy, float* trace, unsigned* trace_count,

@@ -108,7 +108,9 @@ def check(got, ref, lead):
     assert rel(np.triu(got["a"][:lead, :lead]), np.triu(ref["a"][:lead, :lead])) <= 1e-12
 
 
-@pytest.mark.parametrize("waves", [4, 8])
+@pytest.mark.parametrize("waves", [4, pytest.param(8, marks=pytest.mark.skip(
+    reason="8-wave shape switched off: runs of it on a 4096-matrix batch ended in "
+           "host-heap aborts of unidentified cause"))])
 def test_batched_vs_oracle(gpu, oracle, waves):
     import qrdm_amd
     torch = gpu
@@ -196,21 +198,30 @@ def test_batched_nonfinite_and_bad_items(gpu):
 
 
 def test_batched_autotune_keeps_results(gpu, oracle):
-    """Autotuning (launch shape only, nb fixed) leaves the input untouched and
-    the results equal to the oracle's."""
+    """Autotuning (nb among 16, 32, 64 for a plan made with nb <= 0) leaves
+    the input untouched and the results equal to the oracle's at the nb it
+    picked."""
     import qrdm_amd
     torch = gpu
     mats, leads = _mixed_small(21, 24)
-    plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=32)
+    plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=0)
     src = pack(torch, plan, mats)
     before = src.cpu().numpy()
     q = plan.autotune(src)
     say("autotune:", q)
-    assert q["nb"] == 32 and q["waves"] in (4, 8) and q["tuned"] == 2
+    assert q["nb"] in (16, 32, 64) and q["waves"] == 4 and q["tuned"] == 3
     assert np.array_equal(src.cpu().numpy(), before)
     buf, tau, jp, nc, info = run_batch(gpu, plan, src, graph=True)
     for i, a in enumerate(mats):
-        check(unpack(plan, i, buf, tau, jp, nc, info), oracle.dgeqrdm(a, nb=32), leads[i])
+        check(unpack(plan, i, buf, tau, jp, nc, info), oracle.dgeqrdm(a, nb=q["nb"]), leads[i])
+
+
+def test_batched_eight_wave_shape_is_refused(gpu):
+    import qrdm_amd
+    plan = qrdm_amd.BatchPlan([(40, 20)], nb=64)
+    with pytest.raises(ValueError, match="switched off"):
+        plan.set_shape(8)
+    assert plan.query()["waves"] == 4
 
 
 @pytest.mark.slow
