@@ -92,7 +92,23 @@ def cpu_baseline(n_sample=8192, rank_sample=2048):
     q = po.dgeqp3_ref(a)
     t_qp3 = time.perf_counter() - t0
     f = flops(n_sample, n_sample)
+    # the one-thread row, on a smaller matrix of the same family (median of 3)
+    po.lib().oracle_ref_set_threads(1)
+    n1 = 4096
+    a1 = inputs.lowrank_noise(n1, n1, n1 // 4, 78)
+    t1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        po.dgeqrdm(a1, impl="ref")
+        t1.append(time.perf_counter() - t0)
+    t1 = sorted(t1)[1]
+    po.lib().oracle_ref_set_threads(cores)
+    one = {"value": round(flops(n1, n1) / t1 / 1e12, 5), "unit": "TF/s", "cores": 1,
+           "seconds": round(t1, 3),
+           "sample": "reference dgeqrdm, 1 thread, %dx%d low-rank(%d)+1e-10 noise, median of 3"
+                     % (n1, n1, n1 // 4)}
     return {"value": round(f / t_dm / 1e12, 5), "unit": "TF/s", "cores": cores,
+            "one_thread": one,
             "kind": "reference",
             "sample": "reference dgeqrdm (compiled from its sources, OpenBLAS 0.3.28, "
                       "%d threads) on one %dx%d low-rank(%d)+1e-10 noise matrix, 1 run: "
