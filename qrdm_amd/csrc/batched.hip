@@ -152,6 +152,9 @@ __device__ void bt_trailing(double* a, int lda, int j, int m, int kk, int c0, in
     // C -= V W on 16-row blocks: D_u(c, r) = sum_k W(k, 16u + c) V(r, k)
     const double* cbase = a + (size_t)cb * lda;
     for (int rb = j + 16 * wave; rb < m; rb += 16 * NW) {
+      // W is read from LDS in every row block: kept out of the loop (64
+      // doubles per lane), it pushed the kernel past 512 VGPRs into scratch
+      __asm__ volatile("" ::: "memory");
       const int row = rb + li;
       double vk[KMAX / 4];
 #pragma unroll
