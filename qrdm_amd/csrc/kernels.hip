@@ -904,13 +904,34 @@ __device__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, doubl
       const double aii = col[0];
       if (i == 0 && tau_dm > 0.0) thres = tau_dm * fabs(aii);
       if (taui != 0.0) {
-        for (int c = i + 1 + wave; c < fjb; c += nw) {
-          double* cc = P + i + (size_t)c * lda;
-          double s = 0.0;
-          for (int r = lane; r < len; r += 64) s = fma(r == 0 ? 1.0 : col[r], cc[r], s);
-          s = wave_sum(s);
-          const double f = -taui * s;
-          for (int r = lane; r < len; r += 64) cc[r] = fma(r == 0 ? 1.0 : col[r], f, cc[r]);
+        // dlarf_ on the later block columns: wave w takes columns
+        // i+1+w, i+1+w+nw, ..., four at a time (one read of the reflector
+        // for four columns); each column's sums keep their order
+        for (int cq = i + 1 + wave; cq < fjb; cq += 4 * nw) {
+          double* cc[4];
+          bool on[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int c = cq + q * nw;
+            on[q] = c < fjb;
+            cc[q] = P + i + (size_t)(on[q] ? c : cq) * lda;
+          }
+          double s[4] = {0.0, 0.0, 0.0, 0.0};
+          for (int r = lane; r < len; r += 64) {
+            const double v = r == 0 ? 1.0 : col[r];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (on[q]) s[q] = fma(v, cc[q][r], s[q]);
+          }
+          double f[4];
+#pragma unroll
+          for (int q = 0; q < 4; q++) f[q] = -taui * wave_sum(s[q]);
+          for (int r = lane; r < len; r += 64) {
+            const double v = r == 0 ? 1.0 : col[r];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (on[q]) cc[q][r] = fma(v, f[q], cc[q][r]);
+          }
         }
       }
       __syncthreads();
