@@ -144,8 +144,9 @@ typedef struct {
 
 typedef struct qrdm_batch_plan qrdm_batch_plan;
 
-/* Uploads the descriptors and builds the workgroup schedule (matrices in
- * decreasing flop count, each to the least loaded workgroup).  nb: the DM
+/* Uploads the descriptors and the order the workgroups take the matrices in
+ * (decreasing flop count; each workgroup takes the next one from a device
+ * counter, so launches of one plan must not overlap).  nb: the DM
  * block (k_max) of every factorisation, 1..64; nb <= 0 lets
  * qrdm_batch_autotune choose it among 16, 32, 64 (it changes the pivots).
  * Returns NULL on error (text through qrdm_last_error). */

@@ -444,23 +444,36 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
 // sched[sched_off[w] .. sched_off[w + 1]) in order.
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_dm_batch(const BatchItem* __restrict__ items,
-                                                      const int* __restrict__ sched,
-                                                      const int* __restrict__ sched_off,
-                                                      double* A, double* tau, int* jpvt,
-                                                      int* ncols, int* info, BatchParams P) {
+                                                      const int* __restrict__ order, int count,
+                                                      int* queue, double* A, double* tau,
+                                                      int* jpvt, int* ncols, int* info,
+                                                      BatchParams P) {
   __shared__ BatchLds<NW> L;
-  const int b0 = sched_off[blockIdx.x], b1 = sched_off[blockIdx.x + 1];
-  for (int b = b0; b < b1; b++) {
-    const int idx = sched[b];
+  __shared__ int s_next;
+  // Each workgroup takes the next matrix of order (decreasing flop count)
+  // until none is left: list scheduling on the actual durations.  A
+  // matrix's arithmetic does not depend on the workgroup that runs it.
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(&queue[0], 1);
+    __syncthreads();
+    const int b = s_next;
+    if (b >= count) break;  // every thread read the same s_next
+    const int idx = order[b];
     const BatchItem it = items[idx];
     const int r = bt_factor<NW>(it, A, tau, jpvt, ncols, P, L);
     if (threadIdx.x == 0) info[idx] = r;
-    __syncthreads();
+    __syncthreads();  // s_next is read by all before thread 0 takes the next
+  }
+  // the last workgroup out zeroes the queue for the next launch (a graph
+  // replay included): every other workgroup has taken its last index
+  if (threadIdx.x == 0 && atomicAdd(&queue[1], 1) == (int)gridDim.x - 1) {
+    atomicExch(&queue[0], 0);
+    atomicExch(&queue[1], 0);
   }
 }
 
-int launch_dm_batch(int waves, int grid, const void* items, const int* sched,
-                     const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
+int launch_dm_batch(int waves, int grid, const void* items, const int* order, int count,
+                     int* queue, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
                      double tol3z, const void* policy, float* trace, unsigned* trace_count,
                      int trace_cap, const float* ranges, hipStream_t s) {
@@ -472,8 +485,8 @@ int launch_dm_batch(int waves, int grid, const void* items, const int* sched,
   // Only the 4-wave shape is instantiated: runs of k_dm_batch<8> on a
   // 4096-matrix batch ended in host-heap aborts of unidentified cause.
   if (waves != 4) return -1;
-  hipLaunchKernelGGL(k_dm_batch<4>, dim3(grid), dim3(256), 0, s, it, sched, sched_off, A, tau,
-                     jpvt, ncols, info, P);
+  hipLaunchKernelGGL(k_dm_batch<4>, dim3(grid), dim3(256), 0, s, it, order, count, queue, A,
+                     tau, jpvt, ncols, info, P);
   return 0;
 }
 

@@ -604,8 +604,8 @@ struct qrdm_batch_plan {
   int grid = 0;
   std::vector<qrdm_batch_item> items;
   void* d_items = nullptr;
-  int* d_sched = nullptr;
-  int* d_off = nullptr;
+  int* d_sched = nullptr;  // count: the matrices in the order they are taken
+  int* d_queue = nullptr;  // 2: the launch's work counter and finished-workgroup count
   size_t a_ext = 0, tau_ext = 0, col_ext = 0;  // elements the batch spans
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -633,38 +633,17 @@ double qr_flops(int m, int n) {
   return 2.0 * a * b * b - 2.0 * b * b * b / 3.0;
 }
 
-// Longest-processing-time schedule: matrices in decreasing cost, each to the
-// least loaded workgroup (ties: lowest index); each workgroup's list keeps
-// that order.
-void batch_schedule(const std::vector<qrdm_batch_item>& it, int grid, std::vector<int>& sched,
-                    std::vector<int>& off) {
+// The order the workgroups take the matrices in: decreasing flop count
+// (ties: lowest index), so that the last matrices to start are the
+// shortest (list scheduling, longest first).
+std::vector<int> batch_order(const std::vector<qrdm_batch_item>& it) {
   const int cnt = (int)it.size();
   std::vector<int> order(cnt);
   for (int i = 0; i < cnt; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
     return qr_flops(it[x].m, it[x].n) > qr_flops(it[y].m, it[y].n);
   });
-  std::vector<std::vector<int>> lists(grid);
-  std::vector<std::pair<double, int>> heap;  // (load, wg), min-heap
-  for (int w = 0; w < grid; w++) heap.emplace_back(0.0, w);
-  auto cmp = [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
-    return x.first > y.first || (x.first == y.first && x.second > y.second);
-  };
-  std::make_heap(heap.begin(), heap.end(), cmp);
-  for (int i : order) {
-    std::pop_heap(heap.begin(), heap.end(), cmp);
-    auto& top = heap.back();
-    lists[top.second].push_back(i);
-    top.first += qr_flops(it[i].m, it[i].n);
-    std::push_heap(heap.begin(), heap.end(), cmp);
-  }
-  sched.clear();
-  off.assign(grid + 1, 0);
-  for (int w = 0; w < grid; w++) {
-    off[w] = (int)sched.size();
-    sched.insert(sched.end(), lists[w].begin(), lists[w].end());
-  }
-  off[grid] = (int)sched.size();
+  return order;
 }
 
 void batch_release_graph(qrdm_batch_plan* p) {
@@ -678,9 +657,9 @@ void batch_launch(qrdm_batch_plan* p, int waves, int nb, double* a, double* tau,
                   int* nc, int* info, const double* th, int stop, hipStream_t s,
                   float* trace = nullptr, unsigned* trace_count = nullptr, int trace_cap = 0,
                   const float* ranges = nullptr) {
-  if (launch_dm_batch(waves, p->grid, p->d_items, p->d_sched, p->d_off, a, tau, jp, nc, info,
-                      nb, th[0], th[1], stop, th[2], sqrt(kEps), p->policy, trace, trace_count,
-                      trace_cap, ranges, s) != 0)
+  if (launch_dm_batch(waves, p->grid, p->d_items, p->d_sched, p->count, p->d_queue, a, tau, jp,
+                      nc, info, nb, th[0], th[1], stop, th[2], sqrt(kEps), p->policy, trace,
+                      trace_count, trace_cap, ranges, s) != 0)
     throw HipError{hipErrorInvalidValue, "batch launch shape"};
   HIPCHK(hipGetLastError());
 }
@@ -899,14 +878,15 @@ qrdm_batch_plan* qrdm_batch_plan_create(int count, const qrdm_batch_item* items,
     HIPCHK(hipGetDevice(&dev));
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     p->grid = std::max(1, std::min(cus, count));
-    std::vector<int> sched, off;
-    batch_schedule(p->items, p->grid, sched, off);
+    const std::vector<int> order = batch_order(p->items);
     HIPCHK(hipMalloc(&p->d_items, sizeof(qrdm_batch_item) * count));
-    HIPCHK(hipMalloc(&p->d_sched, sizeof(int) * sched.size()));
-    HIPCHK(hipMalloc(&p->d_off, sizeof(int) * off.size()));
+    HIPCHK(hipMalloc(&p->d_sched, sizeof(int) * order.size()));
+    HIPCHK(hipMalloc(&p->d_queue, sizeof(int) * 2));
     HIPCHK(hipMemcpy(p->d_items, items, sizeof(qrdm_batch_item) * count, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(p->d_sched, sched.data(), sizeof(int) * sched.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(p->d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->d_sched, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
+    // zero once here; every launch leaves it zero again
+    HIPCHK(hipMemset(p->d_queue, 0, sizeof(int) * 2));
+    HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
   } catch (const HipError& e) {
     g_err = std::string(e.what) + ": " + hipGetErrorString(e.e);
@@ -922,7 +902,7 @@ void qrdm_batch_plan_destroy(qrdm_batch_plan* p) {
   if (p->cap) (void)hipStreamDestroy(p->cap);
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_sched) (void)hipFree(p->d_sched);
-  if (p->d_off) (void)hipFree(p->d_off);
+  if (p->d_queue) (void)hipFree(p->d_queue);
   delete p;
 }
 
@@ -1052,8 +1032,9 @@ int qrdm_dgeqrdm_batched(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_
     if (!p->exec || !(k == p->key)) {
       batch_release_graph(p);
       HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-      const int lr = launch_dm_batch(p->waves, p->grid, p->d_items, p->d_sched, p->d_off, d_a,
-                                     d_tau, d_jpvt, d_ncols, d_info, p->nb_run, th[0], th[1],
+      const int lr = launch_dm_batch(p->waves, p->grid, p->d_items, p->d_sched, p->count,
+                                     p->d_queue, d_a, d_tau, d_jpvt, d_ncols, d_info, p->nb_run,
+                                     th[0], th[1],
                                      stop_mode, th[2], sqrt(kEps), p->policy, nullptr, nullptr, 0,
                                      nullptr, p->cap);
       HIPCHK(hipStreamEndCapture(p->cap, &p->graph));

@@ -195,15 +195,18 @@ void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n
                        const double* V, int ldv, const double* W, hipStream_t s,
                        const DevState* st);
 // Batched DM-QR (batched.hip): one persistent launch of grid workgroups of
-// 64 * waves threads (waves 4 or 8); items points to device BatchItem
-// records (qrdm_batch_item), workgroup w runs sched[sched_off[w]..sched_off[w+1]).
+// 64 * waves threads; items points to device BatchItem records
+// (qrdm_batch_item); the workgroups take the matrices order[0..count) in
+// turn from the counter queue[0] (queue: two ints, zero before the first
+// launch; the last workgroup out zeroes them again, so launches of one
+// queue must not overlap).
 // policy: a PolicyNet (learned.hip) proposing each step's (tau, delta), or
 // nullptr; trace: per-step training records, or nullptr (ranges: the label
 // normalisation tau_lo, tau_hi, delta_lo, delta_hi).
 constexpr int BATCH_NMAX = 1024;
 // Returns -1 (nothing launched) for a launch shape other than 4 waves.
-int launch_dm_batch(int waves, int grid, const void* items, const int* sched,
-                     const int* sched_off, double* A, double* tau, int* jpvt, int* ncols,
+int launch_dm_batch(int waves, int grid, const void* items, const int* order, int count,
+                     int* queue, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
                      double tol3z, const void* policy, float* trace, unsigned* trace_count,
                      int trace_cap, const float* ranges, hipStream_t s);
