@@ -1,7 +1,7 @@
 // batched.hip -- many independent DM-QR factorisations in one persistent
-// launch: each workgroup factorises whole matrices, one after another, from
-// a precomputed schedule (host longest-processing-time order over the
-// matrices' flop counts), with the DM state of the current matrix in LDS.
+// launch: each workgroup factorises whole matrices, one after another,
+// taking the next one (in decreasing flop count, ordered by the host) from a
+// device counter, with the DM state of the current matrix in LDS.
 //
 // The arithmetic of a step is the plain path's (src/dgeqrdm_work.c:694-787)
 // with the column-by-column panel: the same device routines for the norms
