@@ -440,8 +440,8 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
   return 0;
 }
 
-// Persistent batch kernel: workgroup w factorises the matrices
-// sched[sched_off[w] .. sched_off[w + 1]) in order.
+// Persistent batch kernel: the workgroups factorise the matrices
+// order[0 .. count), each taking the next index from queue[0].
 template <int NW>
 __global__ __launch_bounds__(64 * NW) void k_dm_batch(const BatchItem* __restrict__ items,
                                                       const int* __restrict__ order, int count,
