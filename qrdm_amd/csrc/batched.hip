@@ -375,7 +375,7 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
       __syncthreads();
     }
     // --- dgeqr2_mia with truncation (src/dgeqrdm_work.c:735-738)
-    const int kk = panel_exact_body(a, lda, j, m, fjb, P.tau_dm, 1, tau + j, L.red);
+    const int kk = panel_exact_body<4>(a, lda, j, m, fjb, P.tau_dm, 1, tau + j, L.red);
     __syncthreads();
     // --- dlarft from S = V^T V (src/dgeqrdm_work.c:751-754)
     const int c0 = j + fjb;

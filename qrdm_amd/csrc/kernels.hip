@@ -859,6 +859,9 @@ void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const i
 // st, tau indexed from column 0 (the step writes tau[j..]).
 // The panel's work for the whole workgroup (any size up to 1024 threads;
 // red holds blockDim.x / 64 doubles): returns fjb_cmp, the columns reduced.
+// GROUP: block columns a wave updates at a time (1 for many-wave workgroups,
+// 4 for few waves).
+template <int GROUP>
 __device__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, double tau_dm,
                                 int truncate, double* tau, double* red) {
   const int msub = m - j;
@@ -903,10 +906,19 @@ __device__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, doubl
     if (i + 1 < fjb) {
       const double aii = col[0];
       if (i == 0 && tau_dm > 0.0) thres = tau_dm * fabs(aii);
-      if (taui != 0.0) {
-        // dlarf_ on the later block columns: wave w takes columns
-        // i+1+w, i+1+w+nw, ..., four at a time (one read of the reflector
-        // for four columns); each column's sums keep their order
+      if (GROUP == 1 && taui != 0.0) {
+        // dlarf_ on the later block columns: one column per wave at a time
+        for (int c = i + 1 + wave; c < fjb; c += nw) {
+          double* cc = P + i + (size_t)c * lda;
+          double s = 0.0;
+          for (int r = lane; r < len; r += 64) s = fma(r == 0 ? 1.0 : col[r], cc[r], s);
+          const double f = -taui * wave_sum(s);
+          for (int r = lane; r < len; r += 64) cc[r] = fma(r == 0 ? 1.0 : col[r], f, cc[r]);
+        }
+      } else if (GROUP == 4 && taui != 0.0) {
+        // wave w takes columns i+1+w, i+1+w+nw, ..., four at a time (one
+        // read of the reflector for four columns); each column's sums keep
+        // their order, as in the one-column form
         for (int cq = i + 1 + wave; cq < fjb; cq += 4 * nw) {
           double* cc[4];
           bool on[4];
@@ -950,7 +962,7 @@ __global__ __launch_bounds__(1024) void k_panel_exact(double* A, int lda, int j,
     fjb = st->fjb;
     tau += j;
   }
-  const int i = panel_exact_body(A, lda, j, m, fjb, tau_dm, truncate, tau, red);
+  const int i = panel_exact_body<1>(A, lda, j, m, fjb, tau_dm, truncate, tau, red);
   if (threadIdx.x == 0) {
     st->fjb_cmp = i;
     // dlarf_ inside the panel already updated the rejected columns
