@@ -2,9 +2,11 @@
 // panel.hip combine stages defined in kernels.hip (one launch runs several
 // steps of a DM iteration); mixed.hip adds the reduced-precision trailing
 // update, learned.hip the (tau, delta) network, batched.hip the persistent
-// many-matrix kernel.
+// many-matrix kernel, trailing.hip the DM loop's trailing update merged into
+// one pass over C per step.
 #include "kernels.hip"
 #include "panel.hip"
 #include "mixed.hip"
 #include "learned.hip"
 #include "batched.hip"
+#include "trailing.hip"

@@ -131,14 +131,16 @@ struct Buffers {
   double* R1inv;      // 64 * 64
   double* G2;         // 64 * 64
   double* M;          // 64 * 64
-  double* V;          // mpad * 64
+  double* V;          // 2 * mpad * 64: the dense V of two consecutive steps
   int ldv;
-  double* Wp;  // nsplit * 64 * n
-  double* W;   // 64 * n
+  double* Wp;  // nsplit * 64 * n: partial products of the trailing update
+  double* W;   // 64 * n: W of the three-kernel trailing update (relative to c0)
+  double* Wf;  // 2 * 64 * n: W of two consecutive steps, by absolute column (fused form)
   int* perm;   // kPermInts: (dst, src) pairs, then the panel's candidate ranks
   int* jp;     // n: jpvt on the device during the DM loop
   int* ncols;  // n: fjb_cmp of each DM step
-  unsigned* tickets;  // launch tickets (zero between uses): [0] top-k, [1] selection, [2] reconstruction
+  unsigned* tickets;  // launch tickets (zero between uses): [0] top-k, [1] selection,
+                      // [2] reconstruction, [3] T of the Gram-based panel
   int nsplit;
 };
 
@@ -146,7 +148,8 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   char* p = base;
   const int mpad = ((m + 63) / 64) * 64;
   const int nslab = (m + 31) / 32 + 1;  // slabs of the Gram and fast-panel passes
-  const int nsplit = 16;  // row slabs of the trailing update's first product
+  const int nsplit = 16;  // row slabs of the trailing update's partial products
+  static_assert(nsplit >= 16, "the fused pass uses up to 16 row slabs");
   Buffers t;
   t.st = carve<DevState>(p, 1);
   t.work = carve<double>(p, n);
@@ -162,14 +165,15 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   t.G2 = carve<double>(p, KMAX * KMAX);
   t.M = carve<double>(p, KMAX * KMAX);
   t.ldv = mpad;
-  t.V = carve<double>(p, (size_t)mpad * KMAX);
+  t.V = carve<double>(p, (size_t)2 * mpad * KMAX);
   t.nsplit = nsplit;
   t.Wp = carve<double>(p, (size_t)nsplit * KMAX * n);
   t.W = carve<double>(p, (size_t)KMAX * n);
+  t.Wf = carve<double>(p, (size_t)2 * KMAX * n);
   t.perm = carve<int>(p, kPermInts);
   t.jp = carve<int>(p, n);
   t.ncols = carve<int>(p, n);
-  t.tickets = carve<unsigned>(p, (size_t)(n + 63) / 64 + 1);
+  t.tickets = carve<unsigned>(p, (size_t)(n + 63) / 64 + 4);
   if (b) *b = t;
   return (size_t)(p - base);
 }
@@ -263,7 +267,7 @@ Workspace& workspace_for(int m, int n, Buffers* b) {
   layout_buffers(m, n, ws.base, b);
   // The buffer offsets depend on (m, n): clear the trailing update's ticket
   // counters, which must start every call at zero.
-  HIPCHK(hipMemset(b->tickets, 0, sizeof(unsigned) * ((size_t)(n + 63) / 64 + 1)));
+  HIPCHK(hipMemset(b->tickets, 0, sizeof(unsigned) * ((size_t)(n + 63) / 64 + 4)));
   return ws;
 }
 
@@ -346,8 +350,24 @@ int run_dm(const Args& x) {
     const char* pe = getenv("QRDM_PANEL");
     const bool force_exact = pe && strcmp(pe, "exact") == 0;
     const int fast_rows = force_exact ? INT_MAX : kFastPanelRows;
+    // The FP64 trailing update runs deferred and merged into the next step's
+    // pass over C (trailing.hip); the reduced-precision forms, and
+    // QRDM_TRAILING=3pass, apply each step's update in three kernels.
+    // The merged pass moves C in 32-byte vectors: lda % 4 == 0, A 32-byte
+    // aligned, at least four rows.
+    const char* pt = getenv("QRDM_TRAILING");
+    const bool fused = x.trail == TRAIL_FP64 && !(pt && strcmp(pt, "3pass") == 0) &&
+                       lda % 4 == 0 && m >= 4 && (reinterpret_cast<uintptr_t>(A) & 31) == 0;
+    StepBufs bufs{B.V, fused ? (size_t)B.ldv * KMAX : 0, B.ldv, B.Wf,
+                  fused ? (size_t)KMAX * n : 0, n};
+    if (fused) {
+      // the previous step's V and W of the first step: zero, so that its
+      // deferred update changes nothing
+      HIPCHK(hipMemsetAsync(B.V, 0, sizeof(double) * 2 * (size_t)B.ldv * KMAX, s));
+      HIPCHK(hipMemsetAsync(B.Wf, 0, sizeof(double) * 2 * (size_t)KMAX * n, s));
+    }
     HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
-    launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, s);
+    launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, fused ? 1 : 0, s);
     const int batch = x.trace ? 1 : kBatch;
     int j0 = nfxd, it = 0;
     // the exact panel and dlarft of the step at column jl (RUN_EXACT: they
@@ -355,7 +375,7 @@ int run_dm(const Args& x) {
     auto queue_exact = [&](int jl) {
       int pq = g_prof.begin(PC_PANEL, s);
       launch_panel_exact(A, lda, jl, m, 0, tau_dm, 1, x.tau, B.st, s, RUN_EXACT);
-      launch_vgram_exact(A, lda, jl, m, B.st, B.V, B.ldv, B.gram_part, s);
+      launch_vgram_exact(A, lda, jl, m, B.st, bufs, B.gram_part, s);
       launch_larft(B.gram_part, x.tau, B.st, B.T, s, RUN_EXACT, vgram_exact_slabs(jl, m));
       g_prof.end(pq, s, 0.0);
     };
@@ -363,6 +383,19 @@ int run_dm(const Args& x) {
     // norm_update (src/dgeqrdm_work.c:774-778); the stop test and the next j
     // follow in the step's end
     auto queue_update = [&](int jl) {
+      if (fused) {
+        // C -= V_prev W_prev and the partials V^T C in one pass, then W, the
+        // R12 rows and the norm downdate (the update itself stays deferred)
+        const int ns = fused_slabs(jl, m, jl + 1, n);
+        int pl = g_prof.begin(PC_LARFB, s);
+        launch_fused(A, lda, jl, m, jl + 1, n, bufs, B.Wp, ns, 1, 1, 0, s, B.st);
+        g_prof.end(pl, s, 0.0);
+        int pw = g_prof.begin(PC_WSUM, s);
+        launch_wsum_norm(A, lda, jl, m, n, B.T, B.Wp, ns, bufs, B.work, B.workold, B.st, tol3z,
+                         s);
+        g_prof.end(pw, s, 0.0);
+        return;
+      }
       int pl = g_prof.begin(PC_LARFB, s);
       launch_gemm1_prec(x.trail, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
       g_prof.end(pl, s, 0.0);
@@ -388,19 +421,20 @@ int run_dm(const Args& x) {
       // the kernel reads nc from the state and zero-fills beyond it), then
       // the greedy acceptance and the permutation plan
       launch_gram_select(A, lda, jl, m, n, B.gram_part, B.G, delta, fast_rows, tall ? 0 : 1,
-                         B.perm, B.tickets + 1, B.st, s);
+                         B.perm, B.tickets + 1, B.st, bufs, s);
       g_prof.end(ps, s, 0.0);
       // --- dgeqr2_mia (src/dgeqrdm_work.c:735-738) ------------------------
       int pp = g_prof.begin(PC_PERM, s);
       if (tall)
-        launch_perm_chol1(A, lda, m, B.work, B.jp, B.perm, B.G, tau_dm, B.R1, B.R1inv, B.st, s);
+        launch_perm_chol1(A, lda, m, B.work, B.jp, B.perm, B.G, tau_dm, B.R1, B.R1inv, B.st,
+                          bufs, s);
       else
-        launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, s);
+        launch_apply_perm(A, lda, m, B.work, B.jp, B.perm, B.st, bufs, s);
       g_prof.end(pp, s, 0.0);
       if (tall) {
         int pq = g_prof.begin(PC_PANEL, s);
         launch_panel_fast(A, lda, jl, m, tau_dm, x.tau, B.gram_part, B.R1, B.R1inv, B.G2, B.T,
-                          B.M, B.V, B.ldv, B.tickets + 2, B.st, s);
+                          B.M, bufs, B.tickets + 2, B.st, s);
         g_prof.end(pq, s, 0.0);
       } else {
         queue_exact(jl);
@@ -434,7 +468,7 @@ int run_dm(const Args& x) {
     // plus a batch of KMAX-wide steps stays below min(m, n)), the next batch
     // is queued before its state is read back, so the device does not idle
     // through the readback; its grids are sized from the same lower bound.
-    int q = 0;
+    int q = 0, jend = j0;
     queue_batch(j0, q);
     while (true) {
       const bool ahead = !x.trace && j0 + batch * KMAX < minmn;
@@ -472,12 +506,22 @@ int run_dm(const Args& x) {
         for (int t = 0; t < l.fjb && t < 64; t++) r.sel[t] = h->sel[t];
       }
       it = h->it;
-      if (h->done) break;  // a batch queued ahead returns at once; the final sync drains it
+      if (h->done) {  // a batch queued ahead returns at once; the final sync drains it
+        jend = h->j;
+        break;
+      }
       j0 = h->j;
       if (in_flight)
         q ^= 1;  // the batch queued ahead is the one to wait for next
       else
         queue_batch(j0, q);
+    }
+    if (fused && it > 0) {
+      // the last step's update is still deferred: it reaches the trailing
+      // block when the stop test ended the loop before min(m, n)
+      const int je = jend;
+      launch_fused(A, lda, je, m, je, n, bufs, B.Wp, fused_slabs(je, m, je, n), 1, 0, it & 1, s,
+                   nullptr);
     }
     if (x.nsteps) *x.nsteps = it;
     HIPCHK(hipMemcpyAsync(x.jpvt, B.jp, sizeof(int) * n, hipMemcpyDeviceToHost, s));

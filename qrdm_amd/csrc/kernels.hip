@@ -720,6 +720,90 @@ __device__ void select_wave(const double* Gs, double delta, int m, int n, int fa
 }
 
 
+// The DM step's candidate Gram over rows j..m (cblas_dsyrk,
+// src/dgeqrdm_work.c:376-379) as per-slab partials, one 64-row slab per
+// workgroup.  With a pending update (the previous step's, deferred: st->fused)
+// the candidate columns first receive it on the slab, X -= V_prev(slab rows)
+// W_prev(:, candidates) on the matrix cores, and the updated rows are written
+// back: the cosines, the panel and the later steps read updated columns.
+// Slabs past m (the grid is sized for the lowest j) write zero partials.
+__global__ __launch_bounds__(256) void k_gram_cand(double* A, int lda, int m, double* partial,
+                                                   const DevState* st, StepBufs bufs) {
+  __shared__ double X[KMAX * (GRAM_ROWS + 1)];
+  __shared__ double Vs[KMAX * (GRAM_ROWS + 1)];
+  __shared__ double Ws[KMAX * (KMAX + 1)];
+  if (dm_stopped(st)) return;
+  const int j = st->j;
+  // the greedy takes candidate 0 even when nc is 0 (all norms zero), so it
+  // receives the update too; the cosines read the Gram only when nc > 1
+  const int nc = max(st->nc, 1);
+  const int rs = j + blockIdx.x * GRAM_ROWS;
+  const int tid = threadIdx.x;
+  const bool pend = update_pending(st);
+  constexpr int Q = KMAX * GRAM_ROWS / 256;
+  double xv[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int t = tid + 256 * q, c = t / GRAM_ROWS, r = t % GRAM_ROWS;
+    const bool ok = c < nc && rs + r < m;
+    xv[q] = ok ? A[(size_t)(j + st->cand_idx[c]) * lda + rs + r] : 0.0;
+  }
+  if (pend) {
+    const double* Vp = v_prev(bufs, st);
+    const double* Wq = w_prev(bufs, st);
+    double vv[Q], wv[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = tid + 256 * q, i = t / GRAM_ROWS, r = t % GRAM_ROWS;
+      vv[q] = (rs + r < m) ? Vp[(size_t)i * bufs.ldv + rs + r] : 0.0;
+      const int c = t % KMAX, i2 = t / KMAX;
+      wv[q] = (c < nc) ? Wq[(size_t)i2 * bufs.ldw + j + st->cand_idx[c]] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = tid + 256 * q;
+      X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
+      Vs[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = vv[q];
+      Ws[(t % KMAX) * (KMAX + 1) + t / KMAX] = wv[q];
+    }
+    __syncthreads();
+    // D(r, c) = sum_i V_prev(rs + r, i) W_prev(i, c): wave w rows 16w..16w+15
+    const int lane = tid & 63, wv_ = tid >> 6, g = lane >> 4, li = lane & 15;
+    dbl4 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < KMAX; k0 += 4) {
+      const double a = Vs[(k0 + g) * (GRAM_ROWS + 1) + 16 * wv_ + li];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ws[(16 * u + li) * (KMAX + 1) + k0 + g],
+                                                      acc[u], 0, 0, 0);
+    }
+    // every (row, column) of X belongs to one lane: no barrier before the update
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        X[(16 * u + li) * (GRAM_ROWS + 1) + 16 * wv_ + g + 4 * v] -= acc[u][v];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = tid + 256 * q, c = t / GRAM_ROWS, r = t % GRAM_ROWS;
+      if (c < nc && rs + r < m)
+        A[(size_t)(j + st->cand_idx[c]) * lda + rs + r] = X[c * (GRAM_ROWS + 1) + r];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int t = tid + 256 * q;
+      X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
+    }
+    __syncthreads();
+  }
+  gram_slab_mfma(X, partial + (size_t)blockIdx.x * KMAX * KMAX);
+}
+
 // The DM step's candidate Gram reduce (as k_gram_reduce, RUN_DM), then the
 // last workgroup to finish (atomic ticket, reset for the next launch) runs
 // the greedy acceptance and the step's plan on it (select_wave).
@@ -727,7 +811,8 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
                                                        int nslab, double* G, double delta,
                                                        int m, int n, int fast_rows,
                                                        int exact_queued, int* perm,
-                                                       unsigned* ticket, DevState* st) {
+                                                       unsigned* ticket, DevState* st,
+                                                       StepBufs bufs) {
   __shared__ double part[4][64];
   __shared__ double Gs[KMAX * (KMAX + 1)];
   __shared__ int s_last;
@@ -770,17 +855,23 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
   }
   __syncthreads();
   if (threadIdx.x < 64) select_wave(Gs, delta, m, n, fast_rows, exact_queued, perm, st);
+  if (update_pending(st)) {
+    // the candidates received the deferred update in k_gram_cand
+    double* Wq = w_prev(bufs, st);
+    const int j = st->j, nc = max(st->nc, 1);
+    for (int e = threadIdx.x; e < KMAX * nc; e += blockDim.x)
+      Wq[(size_t)(e & 63) * bufs.ldw + j + st->cand_idx[e >> 6]] = 0.0;
+  }
 }
 
-void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
+void launch_gram_select(double* A, int lda, int j0, int m, int n, double* partial,
                         double* G, double delta, int fast_rows, int exact_queued, int* perm,
-                        unsigned* ticket, DevState* st, hipStream_t s) {
+                        unsigned* ticket, DevState* st, const StepBufs& bufs, hipStream_t s) {
   int nslab = (m - j0 + GRAM_ROWS - 1) / GRAM_ROWS;
   if (nslab < 1) nslab = 1;
-  hipLaunchKernelGGL(k_gram_partial, dim3(nslab), dim3(256), 0, s, A, lda, j0, m, st->cand_idx,
-                     KMAX, &st->nc, partial, st, RUN_DM);
+  hipLaunchKernelGGL(k_gram_cand, dim3(nslab), dim3(256), 0, s, A, lda, m, partial, st, bufs);
   hipLaunchKernelGGL(k_reduce_select, dim3(KMAX * KMAX / 64), dim3(256), 0, s, partial, nslab, G,
-                     delta, m, n, fast_rows, exact_queued, perm, ticket, st);
+                     delta, m, n, fast_rows, exact_queued, perm, ticket, st, bufs);
 }
 
 // ---------------------------------------------------------------------------
@@ -795,10 +886,23 @@ constexpr int PERM_ROWS = 16;  // rows per workgroup: m / 16 workgroups move the
 // Row slab blk of the step's column moves (256 threads); slab 0 also moves
 // the work and jpvt entries.
 __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp,
-                                const int* __restrict__ perm, const DevState* st, int blk) {
+                                const int* __restrict__ perm, const DevState* st, int blk,
+                                int nblk, const StepBufs& bufs) {
   __shared__ double buf[2 * KMAX * PERM_ROWS];
   if (st->done || st->halt == 1) return;  // halt 2: this step's moves still run
   const int npair = st->npair;
+  if (update_pending(st)) {
+    // the deferred update's W columns move with their columns: block blk
+    // takes rows blk, blk + nblk, ... of W (all reads of a row before its
+    // writes; npair <= 2 KMAX <= the block's threads)
+    double* Wq = w_prev(bufs, st);
+    for (int i = blk; i < KMAX; i += nblk) {
+      const int p = threadIdx.x;
+      const double v = (p < npair) ? Wq[(size_t)i * bufs.ldw + perm[2 * p + 1]] : 0.0;
+      __syncthreads();
+      if (p < npair) Wq[(size_t)i * bufs.ldw + perm[2 * p]] = v;
+    }
+  }
   const int rs = blk * PERM_ROWS;
   // npair <= 2 * KMAX: at most 8 elements per thread (256 threads), all
   // loaded before they are stored
@@ -838,16 +942,16 @@ __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp
 
 __global__ __launch_bounds__(256) void k_apply_perm(double* A, int lda, int m, double* work,
                                                     int* jp, const int* __restrict__ perm,
-                                                    const DevState* st) {
-  apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x);
+                                                    const DevState* st, StepBufs bufs) {
+  apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x, gridDim.x, bufs);
 }
 
 int apply_perm_blocks(int m) { return (m + PERM_ROWS - 1) / PERM_ROWS; }
 
 void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
-                       DevState* st, hipStream_t s) {
+                       DevState* st, const StepBufs& bufs, hipStream_t s) {
   hipLaunchKernelGGL(k_apply_perm, dim3((m + PERM_ROWS - 1) / PERM_ROWS), dim3(256), 0, s, A,
-                     lda, m, work, jp, perm, st);
+                     lda, m, work, jp, perm, st, bufs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1011,10 +1115,12 @@ void launch_build_v(const double* A, int lda, int j, int m, int mpad, const DevS
 constexpr int EX_ROWS = 4 * GRAM_ROWS;
 
 __global__ __launch_bounds__(256) void k_vgram_exact(const double* __restrict__ A, int lda,
-                                                     int m, const DevState* st, double* V,
-                                                     int ldv, double* partial) {
+                                                     int m, const DevState* st, StepBufs bufs,
+                                                     double* partial) {
   __shared__ double X[KMAX * (GRAM_ROWS + 1)];
   if (!step_runs(st, RUN_EXACT)) return;
+  double* V = v_cur(bufs, st);
+  const int ldv = bufs.ldv;
   const int j = st->j, kk = st->fjb_cmp;
   const int tid = threadIdx.x;
   dbl4 acc[4];
@@ -1055,10 +1161,10 @@ int vgram_exact_slabs(int j0, int m) {
   return ns > 0 ? ns : 1;
 }
 
-void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st, double* V,
-                        int ldv, double* partial, hipStream_t s) {
+void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st,
+                        const StepBufs& bufs, double* partial, hipStream_t s) {
   hipLaunchKernelGGL(k_vgram_exact, dim3(vgram_exact_slabs(j0, m)), dim3(256), 0, s, A, lda, m,
-                     st, V, ldv, partial);
+                     st, bufs, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -1679,7 +1785,8 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
   hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, s, st, ncols_dev, m, n, enable_stop, eta);
 }
 
-__global__ void k_dm_begin(DevState* st, int j, int minmn, double tau, double delta) {
+__global__ void k_dm_begin(DevState* st, int j, int minmn, double tau, double delta, int fused) {
+  st->fused = fused;
   st->policy_on = 0;
   st->tau_step = tau;
   st->delta_step = delta;
@@ -1693,8 +1800,9 @@ __global__ void k_dm_begin(DevState* st, int j, int minmn, double tau, double de
   st->work = StepWork{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 }
 
-void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, hipStream_t s) {
-  hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(1), 0, s, st, j, minmn, tau, delta);
+void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, int fused,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_dm_begin, dim3(1), dim3(1), 0, s, st, j, minmn, tau, delta, fused);
 }
 
 // ---------------------------------------------------------------------------

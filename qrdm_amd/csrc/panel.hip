@@ -22,7 +22,9 @@
 //         (or 5e-14), as dlarfg_mia does (src/dlarfg.c:126-133,
 //         src/dgeqr2.c:176-177);
 //   k_v2     V2 = P_bottom M (matrix cores), written into the panel and into
-//         the dense V the trailing update reads.
+//         the dense V the trailing update reads, with V^T V per slab;
+//   k_reduce_t  tau and T from V^T V (the compact WY identity), so that the
+//         block reflector is orthogonal for the V stored.
 // Columns the truncation rejects keep their original values; the caller's
 // trailing update then starts at j + fjb_cmp and applies the kept reflectors
 // to them, as dlarf_ did inside the reference's panel.
@@ -485,18 +487,18 @@ __global__ __launch_bounds__(256) void k_perm_chol1(double* A, int lda, int m, d
                                                     int* jp, const int* __restrict__ perm,
                                                     const double* __restrict__ Gc,
                                                     double tau_dm, double* R1, double* R1inv,
-                                                    DevState* st) {
+                                                    DevState* st, StepBufs bufs) {
   if (blockIdx.x == 0)
     chol1_body(Gc, perm + 4 * KMAX, tau_dm, R1, R1inv, st);
   else
-    apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x - 1);
+    apply_perm_rows(A, lda, m, work, jp, perm, st, blockIdx.x - 1, gridDim.x - 1, bufs);
 }
 
 void launch_perm_chol1(double* A, int lda, int m, double* work, int* jp, const int* perm,
                        const double* Gc, double tau_dm, double* R1, double* R1inv, DevState* st,
-                       hipStream_t s) {
+                       const StepBufs& bufs, hipStream_t s) {
   hipLaunchKernelGGL(k_perm_chol1, dim3(1 + apply_perm_blocks(m)), dim3(256), 0, s, A, lda, m,
-                     work, jp, perm, Gc, tau_dm, R1, R1inv, st);
+                     work, jp, perm, Gc, tau_dm, R1, R1inv, st, bufs);
 }
 
 // ---------------------------------------------------------------------------
@@ -953,10 +955,12 @@ __global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__
 // unit lower block on top.
 __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
                                             const double* __restrict__ M, const DevState* st,
-                                            double* V, int ldv) {
+                                            StepBufs bufs, double* gram_part) {
   __shared__ double Ps[PR_ROWS][LDP];
   __shared__ double Ms[KMAX][LDP];
   if (!step_runs(st, RUN_FAST)) return;
+  double* V = v_cur(bufs, st);
+  const int ldv = bufs.ldv;
   const int j = st->j;
   const int k = st->fjb_cmp;
   const int tid = threadIdx.x;
@@ -1004,17 +1008,95 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
       if (c < k && row - j >= k) A[(size_t)(j + c) * lda + row] = x;
     }
   }
+  // the slab's part of V^T V (rows past m are zero in Ps), for T and tau
+  mfma_64(acc, Ps, true, Ps);
+  const int lane = tid & 63, wv = tid >> 6;
+  double* out = gram_part + (size_t)blockIdx.x * KMAX * KMAX;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      out[(16 * wv + (lane >> 4) + 4 * v) + (16 * u + (lane & 15)) * KMAX] = acc[u][v];
+}
+
+// The block reflector of the Gram-based panel from the V it stored: S = V^T V
+// summed from k_v2's slab partials (fixed order), then tau_i = 2 / S_ii and
+// T = (striu(S) + diag(S) / 2)^-1, so that H_1 ... H_k = I - V T V^T holds for
+// the stored V to rounding (the compact WY identity; Joffrain, Low,
+// Quintana-Orti, van de Geijn, van Zee, "Accumulating Householder
+// transformations, revisited", ACM TOMS 32(2), 2006).  The reconstruction's own
+// tau and T agree with these only to about eps times the square of the
+// panel's condition number, and a block reflector that far from orthogonal
+// would carry that error into the trailing matrix.  The inverse runs in the
+// last workgroup to finish (atomic ticket).
+__global__ __launch_bounds__(256) void k_reduce_t(const double* __restrict__ partial, int m,
+                                                  double* Sg, double* tau, double* T_out,
+                                                  unsigned* ticket, const DevState* st) {
+  __shared__ double part[4][64];
+  __shared__ double SX[KMAX][LDP];
+  __shared__ double SY[KMAX][LDP];
+  __shared__ double Y[4][BLK][BLK + 1];
+  __shared__ double rinv[KMAX];
+  __shared__ int s_last;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int j = st->j;
+  const int nslab = (m - j + PR_ROWS - 1) / PR_ROWS;
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;
+  double sacc = 0.0;
+  for (int b0 = q; b0 < nslab; b0 += 32) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int b = b0 + 4 * u;
+      v[u] = partial[(size_t)(b < nslab ? b : q) * KMAX * KMAX + e];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) sacc += (b0 + 4 * u < nslab) ? v[u] : 0.0;
+  }
+  part[q][threadIdx.x & 63] = sacc;
+  __syncthreads();
+  if (q == 0)
+    st_dev(Sg + e, (part[0][threadIdx.x] + part[1][threadIdx.x]) +
+                       (part[2][threadIdx.x] + part[3][threadIdx.x]));
+  handoff_wait();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
+  __syncthreads();
+  if (!s_last) return;
+  const int k = st->fjb_cmp;
+  const int tid = threadIdx.x;
+  for (int e2 = tid; e2 < KMAX * KMAX; e2 += blockDim.x) {
+    const int r = e2 & 63, c = e2 >> 6;  // consecutive threads: consecutive rows
+    double x;
+    if (r < k && c < k) {
+      const double sv = ld_dev(Sg + r + c * KMAX);
+      x = (r < c) ? sv : (r == c ? 0.5 * sv : 0.0);
+    } else {
+      x = (r == c) ? 1.0 : 0.0;
+    }
+    SX[r][c] = x;
+  }
+  lds_barrier();
+  if (tid < k) tau[j + tid] = 1.0 / SX[tid][tid];  // 2 / S_ii
+  lu_inverse<false>(SX, SY, rinv, Y);  // SY = the upper triangular inverse
+  for (int e2 = tid; e2 < KMAX * KMAX; e2 += blockDim.x) {
+    const int r = e2 & 63, c = e2 >> 6;
+    T_out[r + c * KMAX] = (r < k && c < k && r <= c) ? SY[r][c] : 0.0;
+  }
 }
 
 void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
                        double* gram_part, double* R1, double* R1inv, double* G2, double* T,
-                       double* M, double* V, int ldv, unsigned* ticket, DevState* st,
+                       double* M, const StepBufs& bufs, unsigned* ticket, DevState* st,
                        hipStream_t s) {
   const int nslab = (m - j0 + PR_ROWS - 1) / PR_ROWS;
   hipLaunchKernelGGL(k_qgram, dim3(nslab), dim3(256), 0, s, A, lda, m, R1inv, st, gram_part);
   hipLaunchKernelGGL(k_reduce_recon, dim3(KMAX * KMAX / 64), dim3(256), 0, s, gram_part, nslab,
                      G2, R1, R1inv, A, lda, tau_dm, tau, T, M, ticket, st);
-  hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, m, M, st, V, ldv);
+  hipLaunchKernelGGL(k_v2, dim3(nslab), dim3(256), 0, s, A, lda, m, M, st, bufs, gram_part);
+  hipLaunchKernelGGL(k_reduce_t, dim3(KMAX * KMAX / 64), dim3(256), 0, s, gram_part, m, G2, tau,
+                     T, ticket + 1, st);
 }
 
 }  // namespace qrdm

@@ -51,12 +51,44 @@ struct DevState {
                // kernels return until the host resumes the step.  2: the step's
                // column moves still run (set before them); the next step's
                // top-k turns it into 1, which stops the moves too
+  int fused;   // 1: the trailing update is deferred and merged into the next step's pass
+               // (trailing.hip); 0: applied within its own step
   int policy_on;      // 1: the step's tau and delta come from the learned policy
   double tau_step;    // tau and delta of the current step (the defaults, or the
   double delta_step;  // policy's proposal)
   StepRec last;
   StepWork work;
 };
+
+// The DM loop's reflector buffers: the dense V (ldv x KMAX, zero above the
+// panel) and W = T^T V^T C (KMAX x n, by absolute column) of the current step
+// and of the previous one, alternating with the step's parity: step it writes
+// buffer it & 1.  With st->fused the previous step's update C -= V W is still
+// to be applied (trailing.hip).  vstride = 0 / wstride = 0: one buffer.
+struct StepBufs {
+  double* V;
+  size_t vstride;
+  int ldv;
+  double* W;
+  size_t wstride;
+  int ldw;  // n: W(i, c) at W[i * ldw + c]
+};
+__device__ __forceinline__ double* v_cur(const StepBufs& b, const DevState* st) {
+  return b.V + (size_t)(st->it & 1) * b.vstride;
+}
+__device__ __forceinline__ double* v_prev(const StepBufs& b, const DevState* st) {
+  return b.V + (size_t)((st->it & 1) ^ 1) * b.vstride;
+}
+__device__ __forceinline__ double* w_cur(const StepBufs& b, const DevState* st) {
+  return b.W + (size_t)(st->it & 1) * b.wstride;
+}
+__device__ __forceinline__ double* w_prev(const StepBufs& b, const DevState* st) {
+  return b.W + (size_t)((st->it & 1) ^ 1) * b.wstride;
+}
+// Whether the previous step's update is still to be applied.
+__device__ __forceinline__ bool update_pending(const DevState* st) {
+  return st->fused && st->it > 0;
+}
 
 // Value of x in lane l of the wave (l a compile-time or wave-uniform index).
 __device__ __forceinline__ double rdlane(double x, int l) {
@@ -150,13 +182,17 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // candidate ranks into perm[4*KMAX..5*KMAX), the panel form.
 // exact_queued = 0: the step's exact-panel kernels are not queued, and a step
 // that needs them (short panel, fjb < 2) halts for the host to resume it.
-void launch_gram_select(const double* A, int lda, int j0, int m, int n, double* partial,
+// With st->fused and a previous step, the candidate columns first receive
+// that step's deferred update (all rows from j, written back), and their W
+// columns are cleared.
+void launch_gram_select(double* A, int lda, int j0, int m, int n, double* partial,
                         double* G, double delta, int fast_rows, int exact_queued, int* perm,
-                        unsigned* ticket, DevState* st, hipStream_t s);
+                        unsigned* ticket, DevState* st, const StepBufs& bufs, hipStream_t s);
 // Moves the step's columns (st->npair pairs in perm), their work entries and
 // jpvt entries (jp may be null).
+// With a pending update the moved columns' W entries move with them.
 void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const int* perm,
-                       DevState* st, hipStream_t s);
+                       DevState* st, const StepBufs& bufs, hipStream_t s);
 // Column-by-column panel on columns [j, j + fjb) (RUN_ALWAYS: host j, fjb;
 // RUN_EXACT: from st, tau indexed from j).  Sets st->fjb_cmp and st->c0.
 void launch_panel_exact(double* A, int lda, int j, int m, int fjb, double tau_dm,
@@ -170,8 +206,8 @@ void launch_larft(const double* S, const double* tau, const DevState* st, double
 // The DM loop's exact-panel V and its partial Grams (RUN_EXACT; j0 <= j
 // sizes the grid: vgram_exact_slabs(j0, m) partials).
 int vgram_exact_slabs(int j0, int m);
-void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st, double* V,
-                        int ldv, double* partial, hipStream_t s);
+void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState* st,
+                        const StepBufs& bufs, double* partial, hipStream_t s);
 // Block reflector from the left on rows [j, m), columns [c0, n):
 // trans = 1: C := (I - V T V^T)^T C (dgeqrdm's trailing update);
 // trans = 0: C := (I - V T V^T) C.
@@ -242,7 +278,8 @@ void launch_norm_update(const double* A, int lda, int j0, int m, int n, double* 
 void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop, double eta,
                      hipStream_t s);
 // Start of the DM loop at column j.
-void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, hipStream_t s);
+void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, int fused,
+                     hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
 // Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST
@@ -253,14 +290,35 @@ void launch_set_kk(DevState* st, int kk, hipStream_t s);
 // sizes the grids.  Sets st->fjb_cmp and st->c0, or the fallback flag.
 void launch_perm_chol1(double* A, int lda, int m, double* work, int* jp, const int* perm,
                        const double* Gc, double tau_dm, double* R1, double* R1inv, DevState* st,
-                       hipStream_t s);
+                       const StepBufs& bufs, hipStream_t s);
 void launch_panel_fast(double* A, int lda, int j0, int m, double tau_dm, double* tau,
                        double* gram_part, double* R1, double* R1inv, double* G2, double* T,
-                       double* M, double* V, int ldv, unsigned* ticket, DevState* st,
+                       double* M, const StepBufs& bufs, unsigned* ticket, DevState* st,
                        hipStream_t s);
 int apply_perm_blocks(int m);
 // Clears the halt of a step whose exact panel the host queues next.
 void launch_resume(DevState* st, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
+
+// ---- the merged trailing update (trailing.hip) -------------------------------
+// Row slabs of a fused pass over rows [r_lo, m), columns [c0, n) (at most
+// fused_max_slabs()); the partials Wp hold that many KMAX x n blocks.
+int fused_slabs(int r_lo, int m, int c0, int n);
+int fused_max_slabs();
+// One pass over C (rows [r_lo, m), columns [c0, n)): apply: C -= V_prev
+// W_prev; accum: Wp[slab] = V_cur^T C.  With st (the DM loop) r_lo and c0
+// come from st (host values: lower bounds sizing the grid) and the buffers
+// from the step's parity; without, buffer `cur` is the current one.
+void launch_fused(double* A, int lda, int r_lo, int m, int c0, int n, const StepBufs& bufs,
+                  double* Wp, int ns, int apply, int accum, int cur, hipStream_t s,
+                  const DevState* st);
+// The end of a fused step: W = T^T sum_slab Wp for the columns [j + kk, n)
+// (zero below c0), the step's R12 rows C_top - V_top W (written), the norm
+// downdate and recompute of those columns (the recomputed ones get the whole
+// update now and a zero W column), into the current W buffer.  j0 <= j sizes
+// the grid; ns as the fused pass.
+void launch_wsum_norm(double* A, int lda, int j0, int m, int n, const double* T,
+                      const double* Wp, int ns, const StepBufs& bufs, double* work,
+                      double* workold, DevState* st, double tol3z, hipStream_t s);
 
 }  // namespace qrdm

@@ -78,8 +78,17 @@ def main():
         r = po.dgeqrdm(a, impl="ref")
         print("reference dgeqrdm 8192^2 in %.1fs" % (time.time() - t))
         d = np.abs(np.diag(r["a"]))
-        save("config2_ref.npz", seed=inputs.CONFIG2_SEED, info=r["info"], jpvt_lead=r["jpvt"][:2048],
-             ncols=r["ncols"], diag_r=d[:2304].copy(), tau_lead=r["tau"][:2048])
+        # the leading 2048 reflectors (the numerical-rank part): column norms
+        # of V, two seeded projections V^T x, and eight whole V columns
+        lead = 2048
+        V = np.tril(r["a"][:, :lead], -1)
+        x = np.random.default_rng(inputs.CONFIG2_SEED + 1).standard_normal((8192, 2))
+        vcols = np.arange(0, lead, lead // 8)
+        save("config2_ref.npz", seed=inputs.CONFIG2_SEED, info=r["info"], jpvt_lead=r["jpvt"][:lead],
+             ncols=r["ncols"], diag_r=d[:2304].copy(), tau_lead=r["tau"][:lead],
+             r_row_lead=np.triu(r["a"][:lead, :lead]).sum(axis=0),
+             v_colnorms_lead=np.linalg.norm(V, axis=0), v_proj_x=x, v_proj_lead=V.T @ x,
+             v_cols_idx=vcols, v_cols=V[:, vcols].copy())
 
 
 if __name__ == "__main__":

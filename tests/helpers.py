@@ -1,8 +1,9 @@
 """Comparison helpers shared by the parity tests.
 
 Parity bar (the reference's own results on the same inputs):
-  * info, ncols and jpvt identical (jpvt: optionally only a leading part,
-    for inputs whose trailing pivots are decided by rounding noise);
+  * info, ncols and jpvt identical (jpvt and ncols: optionally only a
+    leading part, for inputs whose trailing pivots and truncation points are
+    decided by rounding noise);
   * R (upper trapezoid) within tol normwise, relative to ||R||;
   * each reflector k (V column and tau) within tol after weighting by
     |R_kk| / max|R_ii|: perturbation theory bounds reflector k's forward
@@ -36,8 +37,16 @@ def reflector_errors(got, ref, k):
 
 
 def assert_parity(got, ref, tol=1e-12, lead=None, strict=False):
+    """With lead, only the resolved part is compared: jpvt, R, V and tau of
+    the first lead reflectors, and ncols of the steps that end within them
+    (past it, pivots and truncation points are decided by rounding)."""
     assert got["info"] == ref["info"], (got["info"], ref["info"])
-    assert np.array_equal(got["ncols"], ref["ncols"]), (got["ncols"][:20], ref["ncols"][:20])
+    if lead is None:
+        assert np.array_equal(got["ncols"], ref["ncols"]), (got["ncols"][:20], ref["ncols"][:20])
+    else:
+        nl = int(np.sum(np.cumsum(ref["ncols"]) <= lead))
+        assert np.array_equal(got["ncols"][:nl], ref["ncols"][:nl]), (got["ncols"][:nl],
+                                                                     ref["ncols"][:nl])
     jg, jr = np.asarray(got["jpvt"]), np.asarray(ref["jpvt"])
     if lead is None:
         assert np.array_equal(jg, jr), np.nonzero(jg != jr)[0][:10]
