@@ -359,7 +359,7 @@ int run_dm(const Args& x) {
     const bool fused = x.trail == TRAIL_FP64 && !(pt && strcmp(pt, "3pass") == 0) &&
                        lda % 4 == 0 && m >= 4 && (reinterpret_cast<uintptr_t>(A) & 31) == 0;
     StepBufs bufs{B.V, fused ? (size_t)B.ldv * KMAX : 0, B.ldv, B.Wf,
-                  fused ? (size_t)KMAX * n : 0, n};
+                  fused ? (size_t)KMAX * n : 0};
     if (fused) {
       // the previous step's V and W of the first step: zero, so that its
       // deferred update changes nothing

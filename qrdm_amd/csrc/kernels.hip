@@ -347,10 +347,18 @@ __device__ void step_end_apply(DevState* st, int* ncols_dev, int m, int n, int e
   w.select += 8.0 * nsub + 8.0 * (double)msub * st->kmax;
   w.perm += 16.0 * (double)m * st->npair;
   w.panel += 2.0 * msub * (double)fjb * fjb;
-  w.gemm1 += 2.0 * msub * nct * kk;
-  w.wsum += 2.0 * kk * (double)kk * nct;
-  w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
-  w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
+  if (st->fused) {
+    // the step's fused pass: the previous step's update and this step's
+    // V^T C on rows j..m of columns c0..n; W = T^T sum and the R12 rows
+    w.gemm1 += 2.0 * msub * nct * (double)(kk + st->kk_prev);
+    w.wsum += 4.0 * kk * (double)kk * nct;
+  } else {
+    w.gemm1 += 2.0 * msub * nct * kk;
+    w.wsum += 2.0 * kk * (double)kk * nct;
+    w.gemm2 += 16.0 * msub * nct + 8.0 * msub * kk + 8.0 * kk * nct;
+    w.norm += 8.0 * (double)(kk + 2) * (nsub - kk);
+  }
+  st->kk_prev = kk;
   const int done = step_done(st, m, n, enable_stop, eta);  // from j before the advance
   ncols_dev[st->it] = kk;
   st->it += 1;
@@ -756,15 +764,16 @@ __global__ __launch_bounds__(256) void k_gram_cand(double* A, int lda, int m, do
     for (int q = 0; q < Q; q++) {
       const int t = tid + 256 * q, i = t / GRAM_ROWS, r = t % GRAM_ROWS;
       vv[q] = (rs + r < m) ? Vp[(size_t)i * bufs.ldv + rs + r] : 0.0;
-      const int c = t % KMAX, i2 = t / KMAX;
-      wv[q] = (c < nc) ? Wq[(size_t)i2 * bufs.ldw + j + st->cand_idx[c]] : 0.0;
+      // W(i2, candidate c): a candidate's 64 entries are contiguous
+      const int i2 = t % KMAX, c = t / KMAX;
+      wv[q] = (c < nc) ? Wq[(size_t)(j + st->cand_idx[c]) * KMAX + i2] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int t = tid + 256 * q;
       X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
       Vs[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = vv[q];
-      Ws[(t % KMAX) * (KMAX + 1) + t / KMAX] = wv[q];
+      Ws[(t / KMAX) * (KMAX + 1) + t % KMAX] = wv[q];
     }
     __syncthreads();
     // D(r, c) = sum_i V_prev(rs + r, i) W_prev(i, c): wave w rows 16w..16w+15
@@ -860,7 +869,7 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
     double* Wq = w_prev(bufs, st);
     const int j = st->j, nc = max(st->nc, 1);
     for (int e = threadIdx.x; e < KMAX * nc; e += blockDim.x)
-      Wq[(size_t)(e & 63) * bufs.ldw + j + st->cand_idx[e >> 6]] = 0.0;
+      Wq[(size_t)(j + st->cand_idx[e >> 6]) * KMAX + (e & 63)] = 0.0;
   }
 }
 
@@ -893,14 +902,14 @@ __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp
   const int npair = st->npair;
   if (update_pending(st)) {
     // the deferred update's W columns move with their columns: block blk
-    // takes rows blk, blk + nblk, ... of W (all reads of a row before its
-    // writes; npair <= 2 KMAX <= the block's threads)
+    // takes entries blk, blk + nblk, ... of every moved W column (all reads of
+    // an entry before its writes; npair <= 2 KMAX <= the block's threads)
     double* Wq = w_prev(bufs, st);
     for (int i = blk; i < KMAX; i += nblk) {
       const int p = threadIdx.x;
-      const double v = (p < npair) ? Wq[(size_t)i * bufs.ldw + perm[2 * p + 1]] : 0.0;
+      const double v = (p < npair) ? Wq[(size_t)perm[2 * p + 1] * KMAX + i] : 0.0;
       __syncthreads();
-      if (p < npair) Wq[(size_t)i * bufs.ldw + perm[2 * p]] = v;
+      if (p < npair) Wq[(size_t)perm[2 * p] * KMAX + i] = v;
     }
   }
   const int rs = blk * PERM_ROWS;
@@ -1787,6 +1796,7 @@ void launch_step_end(DevState* st, int* ncols_dev, int m, int n, int enable_stop
 
 __global__ void k_dm_begin(DevState* st, int j, int minmn, double tau, double delta, int fused) {
   st->fused = fused;
+  st->kk_prev = 0;
   st->policy_on = 0;
   st->tau_step = tau;
   st->delta_step = delta;

@@ -853,52 +853,43 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   }
   if (tid < k) tau[tid] = tl[tid];
   lds_barrier();
-  // 7. SR := L^-1 (below the diagonal) and U^-1 (on and above)
-  lu_inverse(SX, SR, rinv, Y);
-  // 8. T = -U L^-T (T V1^T = -U) and M = R^-1 D U^-1, both upper triangular,
-  // computed transposed (lower) so that consecutive lanes store consecutive
-  // rows: the TRI_UU jobs with the tile indices swapped; the tiles below the
-  // diagonal blocks of T and M are zero
+  // 7. SR := U^-1 (on and above the diagonal; zero below)
+  lu_inverse<false>(SX, SR, rinv, Y);
+  // 8. M = R^-1 D U^-1, upper triangular, computed transposed (lower) so that
+  // consecutive lanes store consecutive rows: the TRI_UU jobs with the tile
+  // indices swapped; the tiles below the diagonal blocks are zero.  (T and
+  // tau follow from the V that k_v2 stores, k_reduce_t.)
   {
     const int lane = tid & 63, wv = tid >> 6, li = lane & 15, g = lane >> 4;
     TileJob jt[4];
     const int nj = tri_jobs(TRI_UU, wv, jt);
-    dbl4p at[4], am[4];
+    dbl4p am[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      at[t] = (dbl4p){0.0, 0.0, 0.0, 0.0};
       am[t] = (dbl4p){0.0, 0.0, 0.0, 0.0};
       if (t < nj) {
-        const int cc = BLK * jt[t].tj + li;  // column of T / M (row of the transposed product)
-        const int rr = BLK * jt[t].ti + li;  // row of T / M
+        const int cc = BLK * jt[t].tj + li;  // column of M (row of the transposed product)
+        const int rr = BLK * jt[t].ti + li;  // row of M
         for (int kb = jt[t].k0; kb <= jt[t].k1; kb++) {
 #pragma unroll
           for (int k4 = 0; k4 < BLK; k4 += 4) {
             const int q = BLK * kb + k4 + g;
-            // T^T(c, r) = sum_q L^-1(c, q) U(r, q);  M^T(c, r) = sum_q (D U^-1)(q, c) R^-1(r, q)
-            const double a_t = (q < cc) ? SR[cc][q] : (q == cc ? 1.0 : 0.0);
+            // M^T(c, r) = sum_q (D U^-1)(q, c) R^-1(r, q)
             const double a_m = (q <= cc && q < k) ? dsg[q] * SR[q][cc] : 0.0;
-            const double b_t = (q >= rr) ? SX[rr][q] : 0.0;
-            at[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_t, b_t, at[t], 0, 0, 0);
             am[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_m, SA[rr][q], am[t], 0, 0, 0);
           }
         }
-        // tile rows = columns c of T / M, tile columns = rows r
+        // tile rows = columns c of M, tile columns = rows r
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int c = BLK * jt[t].tj + g + 4 * v, r = BLK * jt[t].ti + li;
-          const bool in = r < k && c < k;
-          T_out[r + c * KMAX] = in ? -at[t][v] : 0.0;
-          M_out[r + c * KMAX] = in ? am[t][v] : 0.0;
+          M_out[r + c * KMAX] = (r < k && c < k) ? am[t][v] : 0.0;
         }
       }
     }
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
       const int r = e & 63, c = e >> 6;
-      if ((r >> 4) > (c >> 4)) {
-        T_out[r + c * KMAX] = 0.0;
-        M_out[r + c * KMAX] = 0.0;
-      }
+      if ((r >> 4) > (c >> 4)) M_out[r + c * KMAX] = 0.0;
     }
   }
   if (tid == 0) {

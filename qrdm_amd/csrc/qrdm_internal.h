@@ -17,7 +17,10 @@ struct StepRec {
 };
 
 // Algorithmic work of the steps run so far, per kernel class (flops or HBM
-// bytes), accumulated on the device for the kernel-time accounting.
+// bytes), accumulated on the device for the kernel-time accounting.  With the
+// merged trailing update (fused), gemm1 holds the fused passes' flops (the
+// previous step's update and this step's V^T C) and wsum those of W = T^T
+// sum Wp and the R12 rows; gemm2 and norm stay zero.
 struct StepWork {
   double select, perm, panel, gemm1, gemm2, norm, wsum;
 };
@@ -53,6 +56,8 @@ struct DevState {
                // top-k turns it into 1, which stops the moves too
   int fused;   // 1: the trailing update is deferred and merged into the next step's pass
                // (trailing.hip); 0: applied within its own step
+  int kk_prev;  // fjb_cmp of the previous step (0 before the first): the width of the
+                // update the fused pass applies
   int policy_on;      // 1: the step's tau and delta come from the learned policy
   double tau_step;    // tau and delta of the current step (the defaults, or the
   double delta_step;  // policy's proposal)
@@ -61,17 +66,17 @@ struct DevState {
 };
 
 // The DM loop's reflector buffers: the dense V (ldv x KMAX, zero above the
-// panel) and W = T^T V^T C (KMAX x n, by absolute column) of the current step
-// and of the previous one, alternating with the step's parity: step it writes
-// buffer it & 1.  With st->fused the previous step's update C -= V W is still
-// to be applied (trailing.hip).  vstride = 0 / wstride = 0: one buffer.
+// panel) and W = T^T V^T C (KMAX x n, by absolute column, column-major: W(i, c)
+// at W[c * KMAX + i]) of the current step and of the previous one,
+// alternating with the step's parity: step it writes buffer it & 1.  With
+// st->fused the previous step's update C -= V W is still to be applied
+// (trailing.hip).  vstride = 0 / wstride = 0: one buffer.
 struct StepBufs {
   double* V;
   size_t vstride;
   int ldv;
   double* W;
   size_t wstride;
-  int ldw;  // n: W(i, c) at W[i * ldw + c]
 };
 __device__ __forceinline__ double* v_cur(const StepBufs& b, const DevState* st) {
   return b.V + (size_t)(st->it & 1) * b.vstride;

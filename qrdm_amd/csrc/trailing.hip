@@ -21,8 +21,9 @@
 // rows and the norms (above), and the candidate columns, which the candidate
 // Gram kernel updates as it reads them (their W columns are then cleared).
 // Columns that move in the step's permutation carry their W column with
-// them.  W is stored by absolute column (64 x n per buffer); V and W of two
-// consecutive steps alternate between two buffers (st->it & 1).
+// them.  W is stored by absolute column (64 x n per buffer, a column's 64
+// entries contiguous); V and W of two consecutive steps alternate between two
+// buffers (st->it & 1).
 //
 // Rounding: each element sees the same products as in the one-step form,
 // accumulated in another order (V W is summed into C on the matrix cores
@@ -34,8 +35,12 @@ namespace qrdm {
 constexpr int FU_CH = 16;         // rows per chunk
 constexpr int FU_NWG = 4;         // waves per workgroup
 constexpr int FU_MAX_SPLIT = 16;  // row slabs of the partial products
-constexpr int FU_VP_LD = 64 + 4;  // LDS row length of the V_s chunk
-constexpr int FU_VN_LD = FU_CH + 4;  // LDS column length of the V_{s+1} chunk
+// LDS layouts of the V chunks (64 banks of 4 bytes), measured free of bank
+// conflicts for the accesses below: V_s by rows of 66 doubles (the product-1
+// operand is read as 16-byte pairs of adjacent columns, a lane's rows 33
+// pairs apart), V_{s+1} by columns of 17 doubles (read as pairs of rows).
+constexpr int FU_VP_LD = 66;
+constexpr int FU_VN_LD = FU_CH + 1;
 
 // Launch shapes of the fused pass: U 16-column blocks per wave (a workgroup
 // covers 64 U columns), C prefetched PF chunks ahead, OCC waves per SIMD.
@@ -92,7 +97,7 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
     int cur, const DevState* st) {
   constexpr int PF = FuShape<U>::PF, NSET = PF + 1;
   constexpr int WC = 16 * U, GC = FU_NWG * WC;
-  __shared__ __align__(32) double VpS[2][FU_CH][FU_VP_LD];
+  __shared__ __align__(16) double VpS[2][FU_CH][FU_VP_LD];
   __shared__ __align__(32) double VnS[2][64][FU_VN_LD];
   if (st) {
     if (dm_stopped(st)) return;
@@ -126,14 +131,15 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
     cok[u] = c < ncol;
     Cc[u] = A + (size_t)(c0 + (cok[u] ? c : (ncol > 0 ? ncol - 1 : 0))) * lda;
   }
-  // the previous step's W tile: wr[s][u] = W(4s + g, column 16u + li)
+  // the previous step's W tile: wr[s][u] = W(16g + s, column 16u + li) (lane
+  // group g takes the k indices 16g..16g+15 of product 1)
   double wr[16][U];
   if (APPLY) {
 #pragma unroll
     for (int s = 0; s < 16; s++)
 #pragma unroll
       for (int u = 0; u < U; u++)
-        wr[s][u] = cok[u] ? Wprev[(size_t)(4 * s + g) * bufs.ldw + c0 + cbw + 16 * u + li] : 0.0;
+        wr[s][u] = cok[u] ? Wprev[(size_t)(c0 + cbw + 16 * u + li) * KMAX + 16 * g + s] : 0.0;
   }
   dbl4 acc[4][U];
 #pragma unroll
@@ -142,27 +148,32 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
     for (int u = 0; u < U; u++) acc[t][u] = (dbl4){0.0, 0.0, 0.0, 0.0};
   // last 4-row group inside the column (rows up to round_up(m, 4) - 1 exist)
   const int rclamp = ((m + 3) & ~3) - 4;
-  // V chunk loader: thread tid brings rows 4*(tid&3).. of V column tid >> 2
-  const int vq = 4 * (tid & 3), vc = tid >> 2;
-  auto load_vv = [&](const double* Vb, int c) -> dbl4 {
+  // V chunk loaders (a thread brings rows vq..vq+3 of column vc).  V_s: wave
+  // w takes rows 8 (w & 1)..+7 of columns 32 (w >> 1)..+31, a lane pair one
+  // column; V_{s+1}: four lanes one column, a wave 16 columns.  Both make the
+  // LDS stores of a wave conflict-free in the layouts above.
+  const int pq = 4 * (2 * (wave & 1) + (lane & 1)), pc = 32 * (wave >> 1) + (lane >> 1);
+  const int nq = 4 * (tid & 3), nc = tid >> 2;
+  auto load_vv = [&](const double* Vb, int c, int vq, int vc) -> dbl4 {
     const int r = G.base + FU_CH * c + vq;
     return *reinterpret_cast<const dbl4*>(Vb + (size_t)vc * ldv + r);
   };
   // rows outside [r_lo, m) enter the products as zero
-  auto mask4 = [&](int c, dbl4 v) -> dbl4 {
+  auto mask4 = [&](int c, int vq, dbl4 v) -> dbl4 {
     const int r = G.base + FU_CH * c + vq;
 #pragma unroll
     for (int q = 0; q < 4; q++) v[q] = (r + q >= r_lo && r + q < m) ? v[q] : 0.0;
     return v;
   };
   auto store_vp = [&](int buf, int c, dbl4 v) {
-    v = mask4(c, v);
-    const int pos = (vc & 3) * 16 + (vc >> 2);  // column 4s + gg stored at gg * 16 + s
+    v = mask4(c, pq, v);
 #pragma unroll
-    for (int q = 0; q < 4; q++) VpS[buf][vq + q][pos] = -v[q];
+    for (int q = 0; q < 4; q++) VpS[buf][pq + q][pc] = -v[q];
   };
   auto store_vn = [&](int buf, int c, dbl4 v) {
-    *reinterpret_cast<dbl4*>(&VnS[buf][vc][vq]) = mask4(c, v);
+    v = mask4(c, nq, v);
+#pragma unroll
+    for (int q = 0; q < 4; q++) VnS[buf][nc][nq + q] = v[q];
   };
   auto load_c = [&](int c, dbl4 (&cc)[U]) {
     const int r = min(G.base + FU_CH * c + 4 * g, rclamp);
@@ -173,9 +184,9 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
   // that output row g + 4v of the 16 x 16 tile is chunk row 4g + v
   const int prow = 4 * (li & 3) + (li >> 2);
   // One chunk: product 1 on cur (C of chunk c) and the store of its updated
-  // rows, then the loads of C for chunk c + PF (into pre, the set stored a
-  // whole chunk earlier) and of V for chunk c + 1, product 2 on cur, the next
-  // V chunk into LDS, and the barrier.
+  // rows, the loads of V for chunk c + 1 and of C for chunk c + PF (into pre,
+  // the set stored a chunk earlier), product 2 on cur, the next V chunk into
+  // LDS, and the barrier.
   auto step = [&](int c, dbl4 (&cur)[U], dbl4 (&pre)[U]) {
     const int buf = (c - ch0) & 1;
     const bool more = c + 1 < ch1;
@@ -185,11 +196,14 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
     if (APPLY) {
       // product 1: C -= Vp W on rows 4g + v of columns 16u + li
 #pragma unroll
-      for (int s = 0; s < 16; s++) {
-        const double a = VpS[buf][prow][g * 16 + s];
+      for (int s = 0; s < 16; s += 2) {
+        const double2 a = *reinterpret_cast<const double2*>(&VpS[buf][prow][16 * g + s]);
 #pragma unroll
         for (int u = 0; u < U; u++)
-          cur[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, wr[s][u], cur[u], 0, 0, 0);
+          cur[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, wr[s][u], cur[u], 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          cur[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, wr[s + 1][u], cur[u], 0, 0, 0);
       }
       if (interior) {
 #pragma unroll
@@ -206,8 +220,8 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
       }
     }
     dbl4 vpn, vnn;
-    if (APPLY) vpn = load_vv(Vp, cn);
-    if (ACCUM) vnn = load_vv(Vn, cn);
+    if (APPLY) vpn = load_vv(Vp, cn, pq, pc);
+    if (ACCUM) vnn = load_vv(Vn, cn, nq, nc);
     if (c + PF < ch1) load_c(c + PF, pre);
     if (ACCUM) {
       if (!interior) {
@@ -239,8 +253,8 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
   };
   dbl4 cs[NSET][U];
   if (ch1 > ch0) {
-    if (APPLY) store_vp(0, ch0, load_vv(Vp, ch0));
-    if (ACCUM) store_vn(0, ch0, load_vv(Vn, ch0));
+    if (APPLY) store_vp(0, ch0, load_vv(Vp, ch0, pq, pc));
+    if (ACCUM) store_vn(0, ch0, load_vv(Vn, ch0, nq, nc));
 #pragma unroll
     for (int p = 0; p < PF; p++)
       if (ch0 + p < ch1) load_c(ch0 + p, cs[p]);
@@ -335,7 +349,8 @@ __global__ __launch_bounds__(256) void k_wsum_norm(double* A, int lda, int m, in
   __shared__ double S1[KMAX][LDP];  // the partials' sum, then W
   __shared__ double S2[KMAX][LDP];  // C's top rows, then R12; C chunks of the recompute
   __shared__ double s_ss[4][KMAX];  // per-wave sums of squares of the recomputed columns
-  __shared__ double s_nrm[KMAX];
+  __shared__ double s_wk[KMAX], s_wo[KMAX];  // work, workold of the tile's columns
+  __shared__ unsigned long long s_rows[4];
   __shared__ int s_rec[KMAX];
   __shared__ int s_any;
   if (dm_stopped(st)) return;
@@ -349,31 +364,58 @@ __global__ __launch_bounds__(256) void k_wsum_norm(double* A, int lda, int m, in
   const double* V = v_cur(bufs, st);
   double* W = w_cur(bufs, st);
   const int ldv = bufs.ldv;
+  // every global read of the prologue in flight together: T, the norms, the
+  // top rows of C and V, and the slab partials (two slabs per round)
+  if (tid < KMAX) {
+    const int col = cb + tid;
+    s_wk[tid] = col < n ? work[col] : 0.0;
+    s_wo[tid] = col < n ? workold[col] : 1.0;
+    s_rec[tid] = 0;
+  }
   load_mat(S0, T);
-  // S1(i, c) = sum_y Wp[y](i, cb + c), all slab loads in flight
-  for (int q = 0; q < KMAX * KMAX / 256; q++) {
-    const int e = tid + 256 * q, i = e >> 6, c = e & 63, col = cb + c;
+  {
+    const int c = tid & 63, i0 = tid >> 6, col = cb + c;
     const bool in = col >= c0 && col < n;
     const int cl = col < n ? col : n - 1;
-    double pv[FU_MAX_SPLIT];
+    double sum[16];
 #pragma unroll
-    for (int y = 0; y < FU_MAX_SPLIT; y++)
-      pv[y] = Wp[((size_t)(y < ns ? y : ns - 1) * KMAX + i) * n + cl];
-    double sum = 0.0;
+    for (int q = 0; q < 16; q++) sum[q] = 0.0;
+    for (int y = 0; y < ns; y += 2) {
+      const int y2 = (y + 1 < ns) ? y + 1 : y;
+      double a0[16], a1[16];
 #pragma unroll
-    for (int y = 0; y < FU_MAX_SPLIT; y++) sum += (y < ns) ? pv[y] : 0.0;
-    S1[i][c] = in ? sum : 0.0;
+      for (int q = 0; q < 16; q++) {
+        const int i = i0 + 4 * q;
+        a0[q] = Wp[((size_t)y * KMAX + i) * n + cl];
+        a1[q] = Wp[((size_t)y2 * KMAX + i) * n + cl];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        sum[q] += a0[q];
+        if (y + 1 < ns) sum[q] += a1[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) S1[i0 + 4 * q][c] = in ? sum[q] : 0.0;
   }
-  if (tid < KMAX) s_rec[tid] = 0;
   lds_barrier();
   dbl4 acc[4];
   mfma_64(acc, S0, true, S1);  // T^T S1
-  lds_barrier();
-  mfma_store(S1, acc);  // S1 = W
+  // V's top rows and C's top rows (rows j..j+kk) while the product runs
+  double vt[KMAX * KMAX / 256], ct[KMAX * KMAX / 256];
+#pragma unroll
   for (int q = 0; q < KMAX * KMAX / 256; q++) {
     const int e = tid + 256 * q, r = e & 63, cc = e >> 6;
-    S0[r][cc] = (r < kk) ? V[(size_t)cc * ldv + j + r] : 0.0;  // V(j + r, cc)
-    S2[r][cc] = (r < kk && cb + cc < n) ? A[(size_t)(cb + cc) * lda + j + r] : 0.0;
+    vt[q] = (r < kk) ? V[(size_t)cc * ldv + j + r] : 0.0;  // V(j + r, cc)
+    ct[q] = (r < kk && cb + cc < n) ? A[(size_t)(cb + cc) * lda + j + r] : 0.0;
+  }
+  lds_barrier();
+  mfma_store(S1, acc);  // S1 = W
+#pragma unroll
+  for (int q = 0; q < KMAX * KMAX / 256; q++) {
+    const int e = tid + 256 * q, r = e & 63, cc = e >> 6;
+    S0[r][cc] = vt[q];
+    S2[r][cc] = ct[q];
   }
   lds_barrier();
   mfma_64(acc, S0, false, S1);  // V_top W
@@ -386,35 +428,37 @@ __global__ __launch_bounds__(256) void k_wsum_norm(double* A, int lda, int m, in
     const int e = tid + 256 * q, r = e & 63, cc = e >> 6;
     if (r < kk && cb + cc < n) A[(size_t)(cb + cc) * lda + j + r] = S2[r][cc];
   }
-  // norm downdate: wave w takes columns 16w..16w+15, lane r holds R12 row r
-  double mx = 0.0;
-  for (int q = 0; q < 16; q++) {
-    const int c = 16 * wv + q, col = cb + c;
-    double nv = 0.0;
-    if (col < n) {
-      double wk = work[col];
-      if (wk != 0.0) {
-        const double x = (lane < kk) ? S2[lane][c] : 0.0;
-        const double sq = wave_sum(x * x);
-        double temp = sqrt(fabs(sq)) / wk;
-        const double d2 = (temp + 1.0) * (1.0 - temp);
-        temp = (0.0 >= d2) ? 0.0 : d2;
-        const double d1 = wk / workold[col];
-        const double temp2 = temp * (d1 * d1);
-        if (temp2 <= tol3z) {
-          if (lane == 0) s_rec[c] = 1;
-        } else {
-          wk *= sqrt(temp);
-          if (lane == 0) work[col] = wk;
-          nv = wk;
-        }
-      }
-    }
-    if (nv == nv) mx = fmax(mx, nv);
+  // norm downdate: the sum of squares of column c's kk R12 entries as four
+  // partial sums (rows p, p + 4, ...; thread p * 64 + c) added in a fixed order
+  {
+    const int c = tid & 63, p = tid >> 6;
+    double sq = 0.0;
+    for (int r = p; r < kk; r += 4) sq = fma(S2[r][c], S2[r][c], sq);
+    s_ss[p][c] = sq;
   }
   if (tid == 0) s_any = 0;
   lds_barrier();
-  if (tid < KMAX && s_rec[tid]) s_any = 1;
+  double mx = 0.0;
+  if (tid < KMAX) {
+    const int c = tid, col = cb + c;
+    const double wk = s_wk[c];
+    if (col < n && wk != 0.0) {
+      const double sq = (s_ss[0][c] + s_ss[1][c]) + (s_ss[2][c] + s_ss[3][c]);
+      double temp = sqrt(fabs(sq)) / wk;
+      const double d2 = (temp + 1.0) * (1.0 - temp);
+      temp = (0.0 >= d2) ? 0.0 : d2;
+      const double d1 = wk / s_wo[c];
+      const double temp2 = temp * (d1 * d1);
+      if (temp2 <= tol3z) {
+        s_rec[c] = 1;
+        s_any = 1;
+      } else {
+        const double nw = wk * sqrt(temp);
+        s_wk[c] = nw;
+        if (nw == nw) mx = nw;
+      }
+    }
+  }
   lds_barrier();
   unsigned long long rows = 0;
   if (s_any) {
@@ -464,22 +508,25 @@ __global__ __launch_bounds__(256) void k_wsum_norm(double* A, int lda, int m, in
       else if (sq >= 1e-280 && sq <= 1e280) nr = sqrt(sq);
       else nr = wave_nrm2(A + (size_t)col * lda + jn, mrem);
       if (lane == 0) {
-        work[col] = nr;
+        s_wk[c] = nr;
         workold[col] = nr;
       }
       if (nr == nr) mx = fmax(mx, nr);
       rows += (unsigned long long)(mrem > 0 ? mrem : 0);
     }
   }
-  // W of the tile's columns (zero for the recomputed ones) into the step's buffer
+  lds_barrier();
+  // the tile's norms, and its W (zero for the recomputed columns), column-major
+  if (tid < KMAX && cb + tid < n) work[cb + tid] = s_wk[tid];
   for (int q = 0; q < KMAX * KMAX / 256; q++) {
-    const int e = tid + 256 * q, i = e >> 6, c = e & 63;
-    if (cb + c < n) W[(size_t)i * bufs.ldw + cb + c] = s_rec[c] ? 0.0 : S1[i][c];
+    const int e = tid + 256 * q, i = e & 63, c = e >> 6;
+    if (cb + c < n) W[(size_t)(cb + c) * KMAX + i] = s_rec[c] ? 0.0 : S1[i][c];
   }
-  if (lane == 0) s_nrm[wv] = (double)rows;
-  block_max_atomic(mx, 0, st);
+  if (lane == 0) s_rows[wv] = rows;
+  // (a lane of wave 0 holds its own column's norm: the wave's maximum goes in)
+  block_max_atomic(wave_max(mx), 0, st);
   if (tid == 0) {
-    const unsigned long long r = (unsigned long long)(s_nrm[0] + s_nrm[1] + s_nrm[2] + s_nrm[3]);
+    const unsigned long long r = s_rows[0] + s_rows[1] + s_rows[2] + s_rows[3];
     if (r) atomicAdd(&st->recomp_rows, r);
   }
 }
