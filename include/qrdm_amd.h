@@ -53,6 +53,24 @@ int dgeqrdm(int matrix_layout, int m, int n, double *a, int lda, int *jpvt, doub
 int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
                      double *d_tau, int *ncols, const double *thres, int nb, void *stream);
 
+/*
+ * qrdm_dgeqrdm_dev that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate during a call (serving loops and
+ * harnesses that forbid hipMalloc).  Same arguments, arithmetic and results
+ * as qrdm_dgeqrdm_dev; its device workspace is a fixed 64 MiB array inside
+ * the library (one per device, held until the process ends), enough for
+ * e.g. m = n = 4096 or m = 16384, n = 2048; a larger problem returns -14
+ * (message on stderr) before any device work.  No device or pinned memory,
+ * events or memset nodes are created: regions that must start at zero are
+ * cleared by a kernel, and the DM loop's state is read back into host memory
+ * of the library, waiting on the stream.  Calls that overlap on one device
+ * share the array and race; the call synchronises the stream, so it cannot
+ * be captured into a graph.
+ */
+int qrdm_dgeqrdm_dev_noalloc(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
+                             double *d_tau, int *ncols, const double *thres, int nb,
+                             void *stream);
+
 /* Precision of the trailing update's two products (qrdm_opts.trailing). */
 #define QRDM_TRAILING_FP64 0   /* FP64 matrix cores (the plain path) */
 #define QRDM_TRAILING_FP32 1   /* FP32 matrix cores, FP32 accumulation */

@@ -42,7 +42,8 @@ class Step(ctypes.Structure):
 
 
 # Every entry point include/qrdm_amd.h declares.
-EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_trace", "qrdm_dgeqrdm_dev_opt",
+EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_noalloc", "qrdm_dgeqrdm_dev_trace",
+           "qrdm_dgeqrdm_dev_opt",
            "qrdm_dgeqrf_dev",
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
            "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
@@ -68,6 +69,7 @@ def lib():
                               _dp, _ip, _dp, ctypes.c_int]
         L.qrdm_dgeqrdm_dev.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V, ctypes.c_int,
                                        _ip, V, _ip, _dp, ctypes.c_int, V]
+        L.qrdm_dgeqrdm_dev_noalloc.argtypes = L.qrdm_dgeqrdm_dev.argtypes
         L.qrdm_dgeqrdm_dev_trace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, V,
                                              ctypes.c_int, _ip, V, _ip, _dp, ctypes.c_int, V,
                                              ctypes.POINTER(Step), ctypes.c_int, _ip]

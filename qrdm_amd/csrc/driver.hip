@@ -222,7 +222,16 @@ struct Args {
   int* nsteps;
   int trail = TRAIL_FP64;  // precision of the trailing update's products
   const void* policy = nullptr;  // learned (tau, delta) network on the device, or null
+  // A caller-given device workspace (at least layout_buffers(m, n) bytes,
+  // 256-byte aligned): the call then allocates nothing -- the state readback
+  // goes to a static host mirror and waits on the stream, regions that must
+  // start at zero are cleared by a kernel, and profiling is suspended.
+  char* scratch = nullptr;
 };
+
+// Host mirror of the device state for calls on a caller-given workspace (two
+// readback slots per device).
+DevState g_host_state[64][2];
 
 int check_args(const Args& x) {
   const double delta = x.thres[0], tau = x.thres[1];
@@ -295,17 +304,41 @@ int run_dm(const Args& x) {
   const int minmn = m < n ? m : n;
 
   Buffers B;
-  Workspace& ws = workspace_for(m, n, &B);
-  DevState* hst = reinterpret_cast<DevState*>(ws.pinned);
+  DevState* hst = nullptr;
+  hipEvent_t* evs = nullptr;  // completion of each readback slot (none: wait on the stream)
+  const bool own_ws = x.scratch == nullptr;
+  if (own_ws) {
+    Workspace& ws = workspace_for(m, n, &B);
+    hst = reinterpret_cast<DevState*>(ws.pinned);
+    evs = ws.ev;
+  } else {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    layout_buffers(m, n, x.scratch, &B);
+    hst = g_host_state[dev & 63];
+    // the launch tickets must start at zero (the workspace keeps the last
+    // call's contents)
+    launch_zero(B.tickets, sizeof(unsigned) * ((size_t)(n + 63) / 64 + 4), s);
+  }
+  auto clear = [&](void* p, size_t bytes) {
+    if (own_ws) HIPCHK(hipMemsetAsync(p, 0, bytes, s));
+    else launch_zero(p, bytes, s);
+  };
+  // profiling records events, which a call on a caller-given workspace does
+  // not create: suspended for it
+  struct ProfPause {
+    bool was, pause;
+    ProfPause(bool p) : was(g_prof.on), pause(p) { if (pause) g_prof.on = false; }
+    ~ProfPause() { if (pause) g_prof.on = was; }
+  } prof_pause(!own_ws);
   double* A = x.a;
 
   // Fixed columns (src/dgeqrdm_work.c:594-610): jpvt != 0 columns go first.
   int nfxd = 0;
-  std::vector<std::pair<int, int>> fixswaps;
   for (int jj = 0; jj < n; jj++) {
     if (x.jpvt[jj] != 0) {
       if (jj != nfxd) {
-        fixswaps.emplace_back(jj, nfxd);
+        launch_swap2(A, lda, m, jj, nfxd, s);
         x.jpvt[jj] = x.jpvt[nfxd];
         x.jpvt[nfxd] = jj + 1;
       } else {
@@ -316,8 +349,7 @@ int run_dm(const Args& x) {
       x.jpvt[jj] = jj + 1;
     }
   }
-  for (auto& pr : fixswaps) launch_swap2(A, lda, m, pr.first, pr.second, s);
-  HIPCHK(hipMemsetAsync(B.st, 0, sizeof(DevState), s));
+  clear(B.st, sizeof(DevState));
   if (nfxd > 0) {
     // Unpivoted blocked QR of the fixed columns, applied to the rest
     // (LAPACKE_dgeqrf + LAPACKE_dormqr, src/dgeqrdm_work.c:618-635).
@@ -363,8 +395,8 @@ int run_dm(const Args& x) {
     if (fused) {
       // the previous step's V and W of the first step: zero, so that its
       // deferred update changes nothing
-      HIPCHK(hipMemsetAsync(B.V, 0, sizeof(double) * 2 * (size_t)B.ldv * KMAX, s));
-      HIPCHK(hipMemsetAsync(B.Wf, 0, sizeof(double) * 2 * (size_t)KMAX * n, s));
+      clear(B.V, sizeof(double) * 2 * (size_t)B.ldv * KMAX);
+      clear(B.Wf, sizeof(double) * 2 * (size_t)KMAX * n);
     }
     HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
     launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, fused ? 1 : 0, s);
@@ -447,7 +479,7 @@ int run_dm(const Args& x) {
     auto readback = [&](int q) {
       launch_step_end(B.st, B.ncols, m, n, enable_stop, eta, s);
       HIPCHK(hipMemcpyAsync(hb[q], B.st, sizeof(DevState), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipEventRecord(ws.ev[q], s));
+      if (evs) HIPCHK(hipEventRecord(evs[q], s));
     };
     auto queue_batch = [&](int jl, int q) {
       // Tall panels: Gram-based Householder reconstruction (panel.hip), which
@@ -461,7 +493,8 @@ int run_dm(const Args& x) {
       readback(q);
     };
     auto wait_slot = [&](int q) {
-      HIPCHK(hipEventSynchronize(ws.ev[q]));
+      if (evs) HIPCHK(hipEventSynchronize(evs[q]));
+      else HIPCHK(hipStreamSynchronize(s));
       return hb[q];
     };
     // While the batch being waited for cannot end the loop (its first column
@@ -710,6 +743,17 @@ void batch_launch(qrdm_batch_plan* p, int waves, int nb, double* a, double* tau,
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Workspace of qrdm_dgeqrdm_dev_noalloc: a device array of fixed size (every
+// device the library runs on holds its own copy until the process ends).  In
+// host code the array's name is not a device address: the address is taken
+// on each device once, on the first call there, with hipGetSymbolAddress, and
+// only that address is used.  Calls that overlap on one device share it and
+// race; a call starts by clearing what must start at zero.
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+static void* g_scratch_ptr[64];  // per device; null before the first call there
+
 extern "C" {
 
 const char* qrdm_version(void) { return "qrdm_amd 0.1 (gfx950)"; }
@@ -791,6 +835,35 @@ int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double* d_a, int lda, int*
                      double* d_tau, int* ncols, const double* thres, int nb, void* stream) {
   return qrdm_dgeqrdm_dev_trace(matrix_layout, m, n, d_a, lda, jpvt, d_tau, ncols, thres, nb,
                                 stream, nullptr, 0, nullptr);
+}
+
+int qrdm_dgeqrdm_dev_noalloc(int matrix_layout, int m, int n, double* d_a, int lda, int* jpvt,
+                             double* d_tau, int* ncols, const double* thres, int nb,
+                             void* stream) {
+  Args x{matrix_layout, m, n, lda, nb, d_a, jpvt, d_tau, ncols, thres,
+         (hipStream_t)stream, nullptr, 0, nullptr};
+  int bad = check_args(x);
+  if (bad) return bad;
+  const size_t need = layout_buffers(m, n, nullptr, nullptr);
+  if (need > kScratchBytes) {
+    fprintf(stderr,
+            "qrdm_dgeqrdm_dev_noalloc: m = %d, n = %d needs %zu bytes of workspace, more than "
+            "the %zu bytes of its scratch array\n",
+            m, n, need, kScratchBytes);
+    return -14;
+  }
+  return guarded_call([&] {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw HipError{hipErrorInvalidDevice, "device index above 63"};
+    if (!g_scratch_ptr[dev]) {
+      void* ptr = nullptr;
+      HIPCHK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = ptr;
+    }
+    x.scratch = static_cast<char*>(g_scratch_ptr[dev]);
+    return run_dm(x);
+  });
 }
 
 int qrdm_dgeqrf_dev(int m, int n, double* d_a, int lda, double* d_tau, void* stream) {

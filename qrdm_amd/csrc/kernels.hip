@@ -1847,4 +1847,20 @@ void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s) {
   hipLaunchKernelGGL(k_swap2, dim3((m + 255) / 256), dim3(256), 0, s, A, lda, m, p, q);
 }
 
+// Zero fill of a workspace region (4-byte words): the clearing that a memset
+// node would do, as an ordinary kernel.
+__global__ __launch_bounds__(256) void k_zero_words(unsigned* p, size_t nw) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < nw; i += (size_t)gridDim.x * 256)
+    p[i] = 0u;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  const size_t nw = bytes / 4;
+  if (nw == 0) return;
+  size_t blocks = (nw + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)blocks), dim3(256), 0, s,
+                     reinterpret_cast<unsigned*>(p), nw);
+}
+
 }  // namespace qrdm
