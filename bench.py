@@ -7,13 +7,16 @@ matrix resident in HBM.  Each timed step starts from the same pristine copy
 (restored on the device before the step's clock starts).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--m M] [--n N]
-                    [--rank R] [--no-cpu-baseline]
+                    [--rank R] [--no-cpu-baseline] [--no-configs]
 
 Prints ONE JSON line (rank 0).  value = whole-factorisation TF/s with
-F = 2 m n^2 - 2 n^3 / 3.  roofline = the trailing update (the dominant
-kernels), timed with HIP events on the library's launch stream during the
-timed steps.  cpu_baseline = the reference compiled from its own sources
-(oracle/_ref, OpenBLAS) on a bounded sample, on this host's cores.
+F = 2 m n^2 - 2 n^3 / 3.  roofline = the trailing update (k_fused, the
+merged pass over C, and k_wsum_norm), its kernels timed with HIP events on
+the library's launch stream during separate event-timed steps.
+cpu_baseline = the reference compiled from its own sources (oracle/_ref,
+OpenBLAS) on the same workload, on this host's cores.  configs = the other
+workloads of the headline metric (1024 x 512 beside the CPU reference and
+dgeqp3, the 16384^2 graded matrix) and the drop-in host-pointer time.
 """
 import argparse
 import json
@@ -28,18 +31,19 @@ METRIC = ("DM-QR time & TF/s, 8192² FP64; mixed/learned vs plain at "
           "||AP-QR||/||A||≤1e-12")
 FP64_MFMA_PEAK_TFS = 78.6
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_gemm1 launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
+# HBM bytes per k_fused launch (FETCH_SIZE + WRITE_SIZE, each scaled by the
 # counter's reading for a transfer of known size in the same run: the initial
 # norm sweep reads the matrix once, the restore copy writes it once), averaged
 # over the launches of the workload (m, n, rank, nb); measured with
-# rocprofv3 --pmc, summary in profiles/k_gemm1_8192_pmc_traffic.md.
-GEMM1_TRAFFIC = {(8192, 8192, 2048, 64): 198066000}
+# rocprofv3 --pmc, summary in profiles/k_fused_8192_pmc_traffic.md.
+FUSED_TRAFFIC = {(8192, 8192, 2048, 64): 366.5e6}
 
 
-def gemm1_algorithmic_bytes(m, n, ncols):
-    """Mean algorithmic HBM bytes of one k_gemm1 launch over a factorisation
-    whose DM steps reduced ncols[it] columns: C (m_sub x (n - c0)) read once,
-    V (m_sub x k) and W (k x (n - c0)) once, c0 = j + k."""
+def fused_algorithmic_bytes(m, n, ncols):
+    """Mean algorithmic HBM bytes of one k_fused launch over a factorisation
+    whose DM steps reduced ncols[it] columns: C (m_sub x (n - c0)) read and
+    written once, the two V blocks (m_sub x 64 each) and the previous W
+    (64 x (n - c0)) read once; c0 = j + k."""
     j, tot, cnt = 0, 0.0, 0
     for k in ncols:
         k = int(k)
@@ -47,7 +51,7 @@ def gemm1_algorithmic_bytes(m, n, ncols):
             break
         msub, nct = m - j, n - j - k
         if nct > 0 and msub > 0:
-            tot += 8.0 * (msub * nct + msub * k + k * nct)
+            tot += 16.0 * msub * nct + 16.0 * msub * 64 + 8.0 * 64 * nct
             cnt += 1
         j += k
     return tot / max(cnt, 1)
@@ -70,11 +74,44 @@ def make_input(torch, m, n, r, seed):
     return a
 
 
-def cpu_baseline(n_sample=8192, rank_sample=2048):
-    """The compiled reference (oracle/_ref) on a bounded sample of the same
-    workload family; returns the dict for the JSON line or None."""
+def host_cpus():
+    """The CPUs this process may run on and what lscpu says about them."""
+    info = {"cpus_available": len(os.sched_getaffinity(0))}
     try:
-        import numpy as np
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        info["sockets"] = int(kv.get("Socket(s)", "0") or 0)
+        info["cores_per_socket"] = int(kv.get("Core(s) per socket", "0") or 0)
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "0") or 0)
+    except Exception:
+        pass
+    return info
+
+
+def median_time(fn, runs, warmup=1):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2], ts
+
+
+def cpu_baseline(n_sample=8192, rank_sample=2048, runs=5):
+    """The compiled reference (oracle/_ref) on the benched workload: one
+    warm-up, then the median of `runs` factorisations on all the CPUs this
+    process may use; dgeqp3 of the same library beside it (median of 3); a
+    one-thread row on 4096^2 of the same family, extrapolated to the benched
+    size by the n^3 flop count."""
+    try:
         from oracle import pyoracle as po
         sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
         import inputs
@@ -82,41 +119,190 @@ def cpu_baseline(n_sample=8192, rank_sample=2048):
         return {"error": "oracle unavailable: %s" % e}
     if not po.ref_available():
         return {"error": "compiled reference (oracle/_ref) not present"}
-    cores = min(os.cpu_count() or 1, 16)
+    cpus = host_cpus()
+    cores = max(1, cpus["cpus_available"])
     po.lib().oracle_ref_set_threads(cores)
     a = inputs.lowrank_noise(n_sample, n_sample, rank_sample, 77)
-    t0 = time.perf_counter()
-    r = po.dgeqrdm(a, impl="ref")
-    t_dm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    q = po.dgeqp3_ref(a)
-    t_qp3 = time.perf_counter() - t0
+    info = []
+    t_dm, ts_dm = median_time(lambda: info.append(po.dgeqrdm(a, impl="ref")["info"]), runs)
     f = flops(n_sample, n_sample)
-    # the one-thread row, on a smaller matrix of the same family (median of 3)
-    po.lib().oracle_ref_set_threads(1)
+    # dgeqp3 (half its flops in matrix-vector products) and the one-thread
+    # reference: one run each on 4096^2 of the same family, scaled to the
+    # benched size by the flop count, to keep the host's share of the run short
     n1 = 4096
     a1 = inputs.lowrank_noise(n1, n1, n1 // 4, 78)
-    t1 = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        po.dgeqrdm(a1, impl="ref")
-        t1.append(time.perf_counter() - t0)
-    t1 = sorted(t1)[1]
+    t_qp3_1, _ = median_time(lambda: info.append(po.dgeqp3_ref(a1)["info"]), 1, warmup=0)
+    t_qp3 = t_qp3_1 * f / flops(n1, n1)
+    po.lib().oracle_ref_set_threads(1)
+    t1, _ = median_time(lambda: po.dgeqrdm(a1, impl="ref"), 1, warmup=0)
     po.lib().oracle_ref_set_threads(cores)
     one = {"value": round(flops(n1, n1) / t1 / 1e12, 5), "unit": "TF/s", "cores": 1,
            "seconds": round(t1, 3),
-           "sample": "reference dgeqrdm, 1 thread, %dx%d low-rank(%d)+1e-10 noise, median of 3"
-                     % (n1, n1, n1 // 4)}
+           "extrapolated_seconds_%dx%d" % (n_sample, n_sample): round(t1 * f / flops(n1, n1), 1),
+           "sample": "reference dgeqrdm, 1 thread, %dx%d low-rank(%d)+1e-10 noise, one run; "
+                     "extrapolated to %dx%d by the flop count" % (n1, n1, n1 // 4, n_sample,
+                                                                  n_sample)}
     return {"value": round(f / t_dm / 1e12, 5), "unit": "TF/s", "cores": cores,
-            "one_thread": one,
-            "kind": "reference",
+            "host": cpus, "one_thread": one, "kind": "reference",
             "sample": "reference dgeqrdm (compiled from its sources, OpenBLAS 0.3.28, "
-                      "%d threads) on one %dx%d low-rank(%d)+1e-10 noise matrix, 1 run: "
-                      "%.2f s; LAPACKE_dgeqp3 on the same matrix: %.2f s (%.5f TF/s)"
-                      % (cores, n_sample, n_sample, rank_sample, t_dm, t_qp3,
-                         f / t_qp3 / 1e12),
-            "info": int(r["info"]) + int(q["info"]),
-            "seconds": round(t_dm, 3), "dgeqp3_seconds": round(t_qp3, 3)}
+                      "%d threads) on the benched workload (%dx%d low-rank(%d)+1e-10 noise), "
+                      "median of %d after one warm-up: %.2f s; LAPACKE_dgeqp3 (same library "
+                      "and threads) on %dx%d of the same family, one run, %.2f s, scaled to "
+                      "%.1f s (%.5f TF/s)"
+                      % (cores, n_sample, n_sample, rank_sample, runs, t_dm, n1, n1, t_qp3_1,
+                         t_qp3, f / t_qp3 / 1e12),
+            "info": int(sum(abs(i) for i in info)),
+            "seconds": round(t_dm, 3), "seconds_all": [round(t, 3) for t in ts_dm],
+            "dgeqp3_seconds_scaled": round(t_qp3, 3)}
+
+
+def config1(torch, qrdm_amd, runs=5):
+    """1024 x 512 N(0,1) (the seeded fixture input): the GPU through the
+    drop-in host-pointer dgeqrdm and through the device-resident form, beside
+    the compiled reference's dgeqrdm and LAPACKE_dgeqp3 on the host (each a
+    median of `runs` after one warm-up)."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import inputs
+    m, n = 1024, 512
+    a = inputs.gaussian(m, n, inputs.CONFIG1_SEED)
+    L = qrdm_amd.lib()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    th = np.array([0.9, 0.15, 0.0])
+
+    def host_call():
+        af = a.copy(order="F")
+        jp = np.zeros(n, np.int32)
+        nc = np.zeros(n, np.int32)
+        t = np.zeros(n)
+        info = L.dgeqrdm(102, m, n, af.ctypes.data_as(dp), m, jp.ctypes.data_as(ip),
+                         t.ctypes.data_as(dp), nc.ctypes.data_as(ip), th.ctypes.data_as(dp), 64)
+        if info != 0:
+            raise RuntimeError("dgeqrdm info %d" % info)
+        return jp
+    t_host, _ = median_time(host_call, runs)
+    jp_gpu = host_call()
+    pristine = torch.from_numpy(np.ascontiguousarray(a.T)).cuda()
+    work = torch.empty_like(pristine)
+    tau = torch.zeros(n, dtype=torch.float64, device="cuda")
+
+    def dev_call():
+        work.copy_(pristine)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        jp = np.zeros(n, np.int32)
+        nc = np.zeros(n, np.int32)
+        info, _ = qrdm_amd.dgeqrdm_dev(work, m, tau, jp, nc, th)
+        torch.cuda.synchronize()
+        if info != 0:
+            raise RuntimeError("dgeqrdm_dev info %d" % info)
+        return time.perf_counter() - t0
+    dev_call()
+    t_dev = sorted(dev_call() for _ in range(runs))[runs // 2]
+    out = {"workload": "dgeqrdm 1024x512 N(0,1), delta=0.9 tau=0.15 nb=64 (seeded fixture input)",
+           "flops": flops(m, n),
+           "gpu_host_pointer_ms": round(1000 * t_host, 3),
+           "gpu_device_resident_ms": round(1000 * t_dev, 3),
+           "gpu_device_resident_tfs": round(flops(m, n) / t_dev / 1e12, 4)}
+    try:
+        from oracle import pyoracle as po
+        if po.ref_available():
+            cores = max(1, len(os.sched_getaffinity(0)))
+            po.lib().oracle_ref_set_threads(cores)
+            ref = {}
+            t_ref, _ = median_time(lambda: ref.update(po.dgeqrdm(a, impl="ref")), runs)
+            t_qp3, _ = median_time(lambda: po.dgeqp3_ref(a), runs)
+            out.update({"cpu_reference_dgeqrdm_ms": round(1000 * t_ref, 3),
+                        "cpu_dgeqp3_ms": round(1000 * t_qp3, 3), "cpu_threads": cores,
+                        "jpvt_identical": bool(np.array_equal(ref["jpvt"], jp_gpu))})
+    except Exception as e:  # pragma: no cover
+        out["cpu_error"] = str(e)
+    return out
+
+
+def config3(torch, qrdm_amd, k=16384, runs=2):
+    """16384^2 graded spectrum A = U diag(s) V^T, s_i = 10^(-8 i / n), U and V
+    from the QR of seeded Gaussians (built on the device): the FP64 path's time
+    (median of `runs` after one warm-up) and its backward error."""
+    import ctypes
+    import numpy as np
+    g = torch.Generator(device="cuda")
+    g.manual_seed(2026)
+    L = qrdm_amd.lib()
+
+    def orthogonal():
+        # Q of the unpivoted QR (the library's dgeqrf and dormqr) of a Gaussian
+        x = torch.randn(k, k, dtype=torch.float64, device="cuda", generator=g)
+        t = torch.zeros(k, dtype=torch.float64, device="cuda")
+        if L.qrdm_dgeqrf_dev(k, k, ctypes.c_void_p(x.data_ptr()), k,
+                             ctypes.c_void_p(t.data_ptr()), None) != 0:
+            raise RuntimeError("dgeqrf_dev")
+        q = torch.eye(k, dtype=torch.float64, device="cuda")
+        if L.qrdm_dormqr_dev(0, k, k, k, ctypes.c_void_p(x.data_ptr()), k,
+                             ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(q.data_ptr()), k,
+                             None) != 0:
+            raise RuntimeError("dormqr_dev")
+        torch.cuda.synchronize()
+        return q.T  # (rows, cols) view of the column-major Q
+    u = orthogonal()
+    v = orthogonal()
+    sv = 10.0 ** (-8.0 * torch.arange(k, dtype=torch.float64, device="cuda") / k)
+    pristine = ((v * sv) @ u.T).contiguous()  # (n, m) row-major = U diag(s) V^T column-major
+    del u, v
+    work = torch.empty_like(pristine)
+    tau = torch.zeros(k, dtype=torch.float64, device="cuda")
+    th = np.array([0.9, 0.15, 0.0])
+    jp = np.zeros(k, np.int32)
+
+    def run():
+        work.copy_(pristine)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        jp[:] = 0
+        nc = np.zeros(k, np.int32)
+        info, _ = qrdm_amd.dgeqrdm_dev(work, k, tau, jp, nc, th)
+        torch.cuda.synchronize()
+        if info != 0:
+            raise RuntimeError("dgeqrdm_dev info %d" % info)
+        return time.perf_counter() - t0, int(np.count_nonzero(nc))
+    run()
+    ts = sorted(run() for _ in range(runs))
+    t, steps = ts[len(ts) // 2]
+    stream = torch.cuda.current_stream().cuda_stream
+    be = backward_error(torch, qrdm_amd, work, pristine, tau, jp, stream)
+    return {"workload": "dgeqrdm %dx%d graded s_i = 10^(-8 i/n), FP64" % (k, k),
+            "ms": round(1000 * t, 2), "tfs": round(flops(k, k) / t / 1e12, 3), "dm_steps": steps,
+            "backward_error": float("%.3g" % be)}
+
+
+def host_pointer_8192(torch, qrdm_amd, pristine, m, n):
+    """The drop-in dgeqrdm on host memory at the benched size: the matrix
+    copied to the device and back inside the call (PCIe-inclusive)."""
+    import ctypes
+    import numpy as np
+    a = np.asfortranarray(pristine.cpu().numpy().T)
+    L = qrdm_amd.lib()
+    dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    th = np.array([0.9, 0.15, 0.0])
+
+    def call():
+        af = a.copy(order="F")
+        jp = np.zeros(n, np.int32)
+        nc = np.zeros(n, np.int32)
+        t = np.zeros(min(m, n))
+        t0 = time.perf_counter()
+        info = L.dgeqrdm(102, m, n, af.ctypes.data_as(dp), m, jp.ctypes.data_as(ip),
+                         t.ctypes.data_as(dp), nc.ctypes.data_as(ip), th.ctypes.data_as(dp), 64)
+        dt = time.perf_counter() - t0
+        if info != 0:
+            raise RuntimeError("dgeqrdm info %d" % info)
+        return dt
+    call()
+    ts = sorted(call() for _ in range(3))
+    return {"workload": "dgeqrdm %dx%d from host memory (pageable numpy array, copies in and "
+                        "out inside the call)" % (m, n),
+            "ms": round(1000 * ts[1], 2), "tfs": round(flops(m, n) / ts[1] / 1e12, 3)}
 
 
 def progress(msg):
@@ -323,6 +509,8 @@ def main():
     ap.add_argument("--rank", type=int, default=2048)
     ap.add_argument("--nb", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 1024x512, 16384^2 and host-pointer runs")
     ap.add_argument("--forms", default="fp32,bf16x3,learned",
                     help="comma list of the added forms to measure beside the plain path "
                          "(fp32, bf16x3, batched, learned; empty for none).  batched and "
@@ -388,15 +576,15 @@ def main():
     progress("timed steps: %.3f ms per factorisation" % ms)
     nsteps_dm = int(np.count_nonzero(last_nc))
     tfs = flops(m, n) / (ms / 1000.0) / 1e12
-    # the same steps again with HIP events around every k_gemm1 launch (each
-    # event record idles the stream for a few microseconds, so these steps
-    # give the kernel's average duration, not the headline time)
-    qrdm_amd.profile(enable=True, reset=True, classes=["larfb_w"])
+    # the same steps again with HIP events around the trailing update's two
+    # kernels (each event record idles the stream for a few microseconds, so
+    # these steps give the kernels' average durations, not the headline time)
+    qrdm_amd.profile(enable=True, reset=True, classes=["larfb_w", "larfb_sum"])
     ms_ev, _ = timed(args.steps)
     prof = qrdm_amd.profile(enable=False, reset=True)
 
     # one more (untimed) factorisation with every kernel class timed, for the
-    # per-class breakdown and the second trailing-update kernel
+    # per-class breakdown
     work.copy_(pristine)
     torch.cuda.synchronize()
     qrdm_amd.profile(enable=True, reset=True)
@@ -404,35 +592,40 @@ def main():
     torch.cuda.synchronize()
     prof_all = qrdm_amd.profile(enable=False, reset=True)
 
-    # dominant kernel: k_gemm1 (partial products V^T C), FP64 MFMA bound; its partner
-    # k_gemm2 (C -= V W) is HBM bound and reported beside it
-    ms_w, launches_w, flops_w = prof["larfb_w"]
-    ms_a, launches_a, bytes_a = prof_all["larfb_apply"]
-    achieved = flops_w / (ms_w / 1000.0) / 1e12 if ms_w > 0 else 0.0
-    avg_w = ms_w / max(launches_w, 1)
-    traffic = GEMM1_TRAFFIC.get((m, n, args.rank, args.nb))
-    alg_bytes = gemm1_algorithmic_bytes(m, n, last_nc)
+    # the trailing update: k_fused (C -= V_prev W_prev with the next step's
+    # V^T C, one read and one write of C) and k_wsum_norm (W = T^T sum, the
+    # R12 rows and the norm downdate); FP64 MFMA bound
+    ms_f, launches_f, flops_f = prof["larfb_w"]
+    ms_s, launches_s, flops_s = prof["larfb_sum"]
+    tot_ms = ms_f + ms_s
+    achieved = (flops_f + flops_s) / (tot_ms / 1000.0) / 1e12 if tot_ms > 0 else 0.0
+    alg_bytes = fused_algorithmic_bytes(m, n, last_nc)
+    bytes_f = alg_bytes * launches_f
     roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_MFMA_PEAK_TFS,
                 "unit": "TF/s", "frac": round(achieved / FP64_MFMA_PEAK_TFS, 4),
-                "traffic": traffic, "algorithmic_bytes": round(alg_bytes),
-                "kernel": "k_gemm1 (trailing update partial products V^T C), %d launches, "
-                          "avg %.4f ms (event-timed steps: %.3f ms per factorisation)"
-                          % (launches_w, avg_w, ms_ev)}
-    achieved_a = bytes_a / (ms_a / 1000.0) / 1e9 if ms_a > 0 else 0.0
-    roofline_apply = {"bound": "hbm", "achieved": round(achieved_a, 1), "peak": HBM_PEAK_GBS,
-                      "unit": "GB/s", "frac": round(achieved_a / HBM_PEAK_GBS, 4),
-                      "kernel": "k_gemm2 (trailing update C -= V W), %d launches, avg %.4f ms "
-                                "(separate profiled factorisation)"
-                                % (launches_a, ms_a / max(launches_a, 1))}
+                "traffic": FUSED_TRAFFIC.get((m, n, args.rank, args.nb)),
+                "algorithmic_bytes": round(alg_bytes),
+                "kernel": "trailing update: k_fused (%d launches, avg %.4f ms) + k_wsum_norm "
+                          "(%d launches, avg %.4f ms); event-timed steps %.3f ms per "
+                          "factorisation" % (launches_f, ms_f / max(launches_f, 1), launches_s,
+                                             ms_s / max(launches_s, 1), ms_ev),
+                "kernels": {
+                    "k_fused": {"ms": round(ms_f, 3), "launches": int(launches_f),
+                                "tflops": round(flops_f / 1e12, 4),
+                                "achieved_tfs": round(flops_f / (ms_f / 1000.0) / 1e12, 3)
+                                if ms_f > 0 else None,
+                                "achieved_GBs": round(bytes_f / (ms_f / 1000.0) / 1e9, 1)
+                                if ms_f > 0 else None},
+                    "k_wsum_norm": {"ms": round(ms_s, 3), "launches": int(launches_s),
+                                    "tflops": round(flops_s / 1e12, 4)}}}
     others = {}
-    for c in ("larfb_w", "larfb_sum", "larfb_apply", "norm_update", "select", "panel",
-              "permute", "init_norms"):
+    for c in ("larfb_w", "larfb_sum", "select", "panel", "permute", "init_norms"):
         t, k, w = prof_all[c]
         entry = {"ms": round(t, 3), "launches": int(k)}
-        if c in ("norm_update", "select", "permute", "init_norms", "larfb_apply") and t > 0:
+        if c in ("select", "permute", "init_norms") and t > 0:
             entry["achieved_GBs"] = round(w / (t / 1000.0) / 1e9, 1)
             entry["frac_hbm"] = round(w / (t / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
-        others[c] = entry
+        others[{"larfb_w": "k_fused", "larfb_sum": "k_wsum_norm"}.get(c, c)] = entry
     line = {"metric": METRIC, "value": round(tfs, 4), "unit": "TF/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -443,7 +636,7 @@ def main():
                        "m": m, "n": n, "rank": args.rank, "nb": args.nb,
                        "dm_steps": nsteps_dm,
                        "kernel_classes_one_factorisation": others},
-            "roofline": roofline, "roofline_apply": roofline_apply}
+            "roofline": roofline}
     progress("profiled steps done")
     forms = [f for f in args.forms.split(",") if f]
     line["forms"] = {}
@@ -456,6 +649,15 @@ def main():
             line["forms"][key] = form_in_child(key, args.batch_loop)
             progress("form %s done" % key)
     progress("forms done")
+    if not args.no_configs:
+        line["configs"] = {"host_pointer_%dx%d" % (m, n): host_pointer_8192(torch, qrdm_amd,
+                                                                            pristine, m, n)}
+        progress("host-pointer run done")
+        line["configs"]["config1_1024x512"] = config1(torch, qrdm_amd)
+        progress("config 1 done")
+        torch.cuda.empty_cache()
+        line["configs"]["config3_16384_fp64"] = config3(torch, qrdm_amd)
+        progress("config 3 done")
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
         progress("cpu baseline done")

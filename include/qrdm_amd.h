@@ -18,7 +18,7 @@ extern "C" {
  *                      double* a, lapack_int lda, lapack_int* jpvt,
  *                      double* tau, lapack_int* ncols, double* thres,
  *                      int DM_procedure);
- * (include/QRDM.h:35-38, src/dgeqrdm.c:5-16, src/dgeqrdm_work.c:420-485).
+ * (include/QRDM.h:19-22, src/dgeqrdm.c:5-16, src/dgeqrdm_work.c:420-485).
  *
  * All arrays are HOST memory owned by the caller and updated in place:
  *   a      lda x n column-major; on exit R (upper trapezoid) and the

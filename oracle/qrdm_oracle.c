@@ -629,7 +629,7 @@ int oracle_dgeqrdm_sched(int matrix_layout, int m, int n, double *a, int lda, in
   return info;
 }
 
-/* Same signature as the reference's dgeqrdm (include/QRDM.h:35-38). */
+/* Same signature as the reference's dgeqrdm (include/QRDM.h:19-22). */
 int oracle_dgeqrdm(int matrix_layout, int m, int n, double *a, int lda, int *jpvt,
                    double *tau, int *ncols, double *thres, int nb) {
   return oracle_dgeqrdm_ex(matrix_layout, m, n, a, lda, jpvt, tau, ncols, thres, nb,

@@ -35,19 +35,21 @@ typedef struct {
   int held;
 } buf_t;
 
-static int get_buf(PyObject* o, buf_t* b, const char* name) {
+/* out != 0: the library writes the buffer, which must then be writable (a
+ * read-only buffer is refused rather than written through). */
+static int get_buf_rw(PyObject* o, buf_t* b, const char* name, int out) {
   b->held = 0;
-  if (PyObject_GetBuffer(o, &b->view, PyBUF_WRITABLE | PyBUF_ANY_CONTIGUOUS) != 0) {
+  if (PyObject_GetBuffer(o, &b->view, (out ? PyBUF_WRITABLE : 0) | PyBUF_ANY_CONTIGUOUS) != 0) {
     PyErr_Clear();
-    if (PyObject_GetBuffer(o, &b->view, PyBUF_ANY_CONTIGUOUS) != 0) {
-      PyErr_Format(PyExc_TypeError, "QRDM: argument '%s' must expose a contiguous buffer",
-                   name);
-      return -1;
-    }
+    PyErr_Format(PyExc_TypeError, "QRDM: argument '%s' must expose a contiguous%s buffer", name,
+                 out ? " writable" : "");
+    return -1;
   }
   b->held = 1;
   return 0;
 }
+#define get_buf(o, b, name) get_buf_rw(o, b, name, 1)
+#define get_in_buf(o, b, name) get_buf_rw(o, b, name, 0)
 
 static void rel(buf_t* b) {
   if (b->held) PyBuffer_Release(&b->view);
@@ -72,7 +74,7 @@ static PyObject* py_qrdm(PyObject* self, PyObject* args) {
   if (get_buf(ojp, &jp, "jpvt")) { rel(&A); return NULL; }
   if (get_buf(otau, &tau, "tau")) { rel(&A); rel(&jp); return NULL; }
   if (get_buf(onc, &nc, "ncols")) { rel(&A); rel(&jp); rel(&tau); return NULL; }
-  if (get_buf(oth, &th, "thres")) { rel(&A); rel(&jp); rel(&tau); rel(&nc); return NULL; }
+  if (get_in_buf(oth, &th, "thres")) { rel(&A); rel(&jp); rel(&tau); rel(&nc); return NULL; }
   int out;
   Py_BEGIN_ALLOW_THREADS
   out = dgeqrdm(layout, m, n, (double*)A.view.buf, lda, (int*)jp.view.buf, (double*)tau.view.buf,
@@ -133,8 +135,8 @@ static PyObject* py_dormqr(PyObject* self, PyObject* args) {
   if (!PyArg_ParseTuple(args, "iiiiOiOOi", &layout, &m, &n, &k, &oA, &lda, &otau, &oC, &ldc))
     return NULL;
   buf_t A, tau, C;
-  if (get_buf(oA, &A, "A")) return NULL;
-  if (get_buf(otau, &tau, "tau")) { rel(&A); return NULL; }
+  if (get_in_buf(oA, &A, "A")) return NULL;
+  if (get_in_buf(otau, &tau, "tau")) { rel(&A); return NULL; }
   if (get_buf(oC, &C, "C")) { rel(&A); rel(&tau); return NULL; }
   int out;
   double* a = (double*)A.view.buf;

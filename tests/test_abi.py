@@ -40,6 +40,33 @@ def test_python_module_surface():
     assert QRDM.init() is None
 
 
+def test_python_module_refuses_read_only_outputs():
+    """The arrays the library writes (A, jpvt, tau, ncols; C of DORMQR) must
+    be writable: a read-only one raises TypeError before any device work;
+    thres (read only by the library) may be read-only."""
+    import QRDM
+    a = np.zeros(20)
+    jp = np.zeros(4, np.int32)
+    t = np.zeros(4)
+    nc = np.zeros(4, np.int32)
+    th = np.array([0.9, 0.15, 0.0])
+    for name in ("a", "jp", "t", "nc"):
+        args = dict(a=a.copy(), jp=jp.copy(), t=t.copy(), nc=nc.copy())
+        args[name].setflags(write=False)
+        with pytest.raises(TypeError, match="writable"):
+            QRDM.QRDM(102, 5, 4, args["a"], 5, args["jp"], args["t"], args["nc"], th, 8)
+    c = np.zeros(20)
+    c.setflags(write=False)
+    with pytest.raises(TypeError, match="writable"):
+        QRDM.DORMQR(102, 5, 4, 2, a, 5, t, c, 5)
+    th_ro = th.copy()
+    th_ro.setflags(write=False)
+    with pytest.raises(TypeError, match="writable"):  # the read-only A is what is refused
+        a_ro = a.copy()
+        a_ro.setflags(write=False)
+        QRDM.QRDM(102, 5, 4, a_ro, 5, jp, t, nc, th_ro, 8)
+
+
 @pytest.mark.parametrize("bad", [
     dict(layout=100), dict(m=0), dict(n=-1), dict(lda=3), dict(delta=1.5),
     dict(delta=-0.1), dict(tau=2.0), dict(nb=0)])
