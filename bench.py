@@ -378,8 +378,10 @@ def backward_error(torch, qrdm_amd, work, pristine, tau, jp, stream):
 def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, stream, ms_plain,
                    which):
     """The plain FP64 path beside the FP32 and split-BF16x3 trailing updates
-    on the same matrix: time per factorisation (same timing as the headline,
-    no events) and the backward error of one factorisation of each."""
+    (scheduled: FP64 until the remaining block's norm bound allows the lower
+    precision within 1e-12; _all: the lower precision in every step) on the
+    same matrix: time per factorisation (same timing as the headline, no
+    events) and the backward error of one factorisation of each."""
     m, n = args.m, args.n
     out = {}
     for form in ["fp64"] + list(which):
@@ -511,9 +513,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 1024x512, 16384^2 and host-pointer runs")
-    ap.add_argument("--forms", default="fp32,bf16x3,learned",
+    ap.add_argument("--forms", default="fp32,bf16x3,fp32_all,learned",
                     help="comma list of the added forms to measure beside the plain path "
-                         "(fp32, bf16x3, batched, learned; empty for none).  batched and "
+                         "(fp32, bf16x3: the scheduled reduced-precision trailing updates; "
+                         "fp32_all, bf16x3_all: the reduced precision in every step; "
+                         "batched, learned; empty for none).  batched and "
                          "learned run in child processes; batched is not in the default list: "
                          "its child ended in a host-heap abort whose cause is not identified "
                          "(the batched form is measured by tests/test_gpu_batched.py)")
@@ -643,7 +647,8 @@ def main():
     if forms:  # the plain path's own backward error beside the forms
         line["forms"] = trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau,
                                        last_jp, stream, ms,
-                                       [f for f in forms if f in ("fp32", "bf16x3")])
+                                       [f for f in forms if f in ("fp32", "bf16x3", "fp32_all",
+                                                                   "bf16x3_all")])
     for key in ("batched", "learned"):
         if key in forms:
             line["forms"][key] = form_in_child(key, args.batch_loop)

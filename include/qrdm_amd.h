@@ -71,10 +71,17 @@ int qrdm_dgeqrdm_dev_noalloc(int matrix_layout, int m, int n, double *d_a, int l
                              double *d_tau, int *ncols, const double *thres, int nb,
                              void *stream);
 
-/* Precision of the trailing update's two products (qrdm_opts.trailing). */
+/* Precision of the trailing update's two products (qrdm_opts.trailing).
+ * FP32 and BF16X3 keep the FP64 products until the remaining trailing block
+ * is small enough for the lower precision to stay within 1e-12 of A
+ * (maxnrm * sqrt(n - j) <= 1e-12 * max_initial_column_norm / (64 u_low),
+ * u_low = 2^-24 for FP32, 2^-17 for BF16X3), then use the lower one; the
+ * _ALL forms use it in every step. */
 #define QRDM_TRAILING_FP64 0   /* FP64 matrix cores (the plain path) */
-#define QRDM_TRAILING_FP32 1   /* FP32 matrix cores, FP32 accumulation */
-#define QRDM_TRAILING_BF16X3 2 /* split BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+#define QRDM_TRAILING_FP32 1   /* FP32 matrix cores, FP32 accumulation, scheduled */
+#define QRDM_TRAILING_BF16X3 2 /* split BF16 (hi*hi + hi*lo + lo*hi), FP32 accumulation, scheduled */
+#define QRDM_TRAILING_FP32_ALL 3   /* FP32 in every step */
+#define QRDM_TRAILING_BF16X3_ALL 4 /* split BF16 in every step */
 
 /* Options of qrdm_dgeqrdm_dev_opt; a NULL pointer means all defaults. */
 typedef struct {
@@ -119,6 +126,7 @@ typedef struct {
   int sel[64];   /* accepted columns relative to j, in acceptance order */
   double tau;    /* the step's tau and delta (thres, or the learned proposal) */
   double delta;
+  int prec;      /* the step's trailing-update precision (QRDM_TRAILING_FP64/FP32/BF16X3) */
 } qrdm_step;
 
 /*
@@ -133,9 +141,11 @@ int qrdm_dgeqrdm_dev_trace(int matrix_layout, int m, int n, double *d_a, int lda
 
 /*
  * qrdm_dgeqrdm_dev_trace with options (opts may be NULL).  No counterpart in
- * the reference, which has only the FP64 path: the reduced-precision
- * trailing updates trade the 1e-12 agreement with the FP64 path for speed
- * (backward error about the FP32 unit roundoff times the matrix's norm).
+ * the reference, which has only the FP64 path.  The scheduled reduced-
+ * precision forms keep the backward error within 1e-12 (they switch only
+ * where the remaining block allows it, and may never switch); the _ALL
+ * forms trade that agreement for speed (backward error about the lower
+ * precision's unit roundoff times the matrix's norm).
  */
 int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double *d_a, int lda, int *jpvt,
                          double *d_tau, int *ncols, const double *thres, int nb,

@@ -23,7 +23,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("trailing", ctypes.c_int), ("policy", ctypes.c_void_p)]
 
 
-TRAILING = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+TRAILING = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp32_all": 3, "bf16x3_all": 4}
 
 
 class BatchItem(ctypes.Structure):
@@ -38,7 +38,7 @@ class Step(ctypes.Structure):
     _fields_ = [("j", ctypes.c_int), ("kmax", ctypes.c_int), ("nc", ctypes.c_int),
                 ("fjb", ctypes.c_int), ("fjb_cmp", ctypes.c_int), ("nswap", ctypes.c_int),
                 ("panel", ctypes.c_int), ("maxnrm", ctypes.c_double), ("sel", ctypes.c_int * 64),
-                ("tau", ctypes.c_double), ("delta", ctypes.c_double)]
+                ("tau", ctypes.c_double), ("delta", ctypes.c_double), ("prec", ctypes.c_int)]
 
 
 # Every entry point include/qrdm_amd.h declares.
@@ -125,7 +125,10 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
     thres: float64 numpy array [delta, tau(, eta)].  stream: hipStream_t as
     an int (torch.cuda.current_stream().cuda_stream), None = default stream.
     trailing: precision of the trailing update's products, "fp64" (the
-    plain path), "fp32" or "bf16x3" (qrdm_dgeqrdm_dev_opt).  policy: a
+    plain path); "fp32" or "bf16x3" (FP64 until the remaining block's norm
+    bound lets the lower precision stay within 1e-12 of A, then the lower
+    one); "fp32_all" or "bf16x3_all" (the lower precision in every step)
+    (qrdm_dgeqrdm_dev_opt).  policy: a
     device tensor holding a qrdm_policy (qrdm_amd.policy), or None.
     Returns (info, steps), steps = per-step records when trace is set.
     """
@@ -154,7 +157,7 @@ def dgeqrdm_dev(a, m, tau, jpvt, ncols, thres, nb=64, stream=None, trace=False,
             return info, None
         steps = [dict(j=t.j, kmax=t.kmax, nc=t.nc, fjb=t.fjb, fjb_cmp=t.fjb_cmp, nswap=t.nswap,
                       panel=t.panel, maxnrm=t.maxnrm, sel=list(t.sel[:min(t.fjb, 64)]),
-                      tau=t.tau, delta=t.delta)
+                      tau=t.tau, delta=t.delta, prec=t.prec)
                  for t in tr[:ns.value]]
         return info, steps
     return L.qrdm_dgeqrdm_dev(*args), None

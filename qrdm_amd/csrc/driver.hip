@@ -383,23 +383,42 @@ int run_dm(const Args& x) {
     const bool force_exact = pe && strcmp(pe, "exact") == 0;
     const int fast_rows = force_exact ? INT_MAX : kFastPanelRows;
     // The FP64 trailing update runs deferred and merged into the next step's
-    // pass over C (trailing.hip); the reduced-precision forms, and
+    // pass over C (trailing.hip); the reduced-precision products, and
     // QRDM_TRAILING=3pass, apply each step's update in three kernels.
     // The merged pass moves C in 32-byte vectors: lda % 4 == 0, A 32-byte
     // aligned, at least four rows.
     const char* pt = getenv("QRDM_TRAILING");
-    const bool fused = x.trail == TRAIL_FP64 && !(pt && strcmp(pt, "3pass") == 0) &&
-                       lda % 4 == 0 && m >= 4 && (reinterpret_cast<uintptr_t>(A) & 31) == 0;
-    StepBufs bufs{B.V, fused ? (size_t)B.ldv * KMAX : 0, B.ldv, B.Wf,
-                  fused ? (size_t)KMAX * n : 0};
-    if (fused) {
+    bool fused_now = !(pt && strcmp(pt, "3pass") == 0) && lda % 4 == 0 && m >= 4 &&
+                     (reinterpret_cast<uintptr_t>(A) & 31) == 0;
+    StepBufs bufs{B.V, fused_now ? (size_t)B.ldv * KMAX : 0, B.ldv, B.Wf,
+                  fused_now ? (size_t)KMAX * n : 0};
+    // Reduced-precision forms (x.trail): the trailing update stays FP64 until
+    // the remaining block is small enough for the low precision's rounding to
+    // stay below 1e-12 of A -- maxnrm sqrt(n - j) <= 1e-12 maxnrm0 /
+    // (64 u_low), maxnrm0 the largest initial column norm (a lower bound of
+    // ||A||), 64 the products' accumulation length, u_low 2^-24 (FP32) or
+    // 2^-17 (split BF16) -- and from then on runs in x.trail in the
+    // three-kernel form.  The host decides between batches (at a batch
+    // boundary with nothing queued behind it).
+    const int low = (x.trail == QRDM_TRAILING_FP32 || x.trail == QRDM_TRAILING_FP32_ALL)
+                        ? TRAIL_FP32 : TRAIL_BF16X3;
+    const bool scheduled = x.trail == QRDM_TRAILING_FP32 || x.trail == QRDM_TRAILING_BF16X3;
+    const bool always_low = x.trail >= QRDM_TRAILING_FP32_ALL;
+    int prec_now = always_low ? low : TRAIL_FP64;
+    if (always_low) {
+      fused_now = false;
+      bufs = StepBufs{B.V, 0, B.ldv, B.Wf, 0};
+    }
+    bool switch_wanted = false;
+    const double low_bound = 1e-12 * maxnrm / (64.0 * (low == TRAIL_FP32 ? 0x1p-24 : 0x1p-17));
+    if (fused_now) {
       // the previous step's V and W of the first step: zero, so that its
       // deferred update changes nothing
       clear(B.V, sizeof(double) * 2 * (size_t)B.ldv * KMAX);
       clear(B.Wf, sizeof(double) * 2 * (size_t)KMAX * n);
     }
     HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
-    launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, fused ? 1 : 0, s);
+    launch_dm_begin(B.st, nfxd, minmn, tau_dm, delta, fused_now ? 1 : 0, s);
     const int batch = x.trace ? 1 : kBatch;
     int j0 = nfxd, it = 0;
     // the exact panel and dlarft of the step at column jl (RUN_EXACT: they
@@ -415,7 +434,7 @@ int run_dm(const Args& x) {
     // norm_update (src/dgeqrdm_work.c:774-778); the stop test and the next j
     // follow in the step's end
     auto queue_update = [&](int jl) {
-      if (fused) {
+      if (fused_now) {
         // C -= V_prev W_prev and the partials V^T C in one pass, then W, the
         // R12 rows and the norm downdate (the update itself stays deferred)
         const int ns = fused_slabs(jl, m, jl + 1, n);
@@ -429,13 +448,13 @@ int run_dm(const Args& x) {
         return;
       }
       int pl = g_prof.begin(PC_LARFB, s);
-      launch_gemm1_prec(x.trail, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
+      launch_gemm1_prec(prec_now, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.Wp, B.nsplit, s, B.st);
       g_prof.end(pl, s, 0.0);
       int pw = g_prof.begin(PC_WSUM, s);
       launch_wsum(jl, m, jl + 1, n, B.T, 1, B.Wp, B.W, B.nsplit, s, B.st);
       g_prof.end(pw, s, 0.0);
       int pa = g_prof.begin(PC_APPLY, s);
-      launch_apply_prec(x.trail, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.W, s, B.st);
+      launch_apply_prec(prec_now, A, lda, jl, m, jl + 1, n, B.V, B.ldv, B.W, s, B.st);
       g_prof.end(pa, s, 0.0);
       int pn = g_prof.begin(PC_NORM, s);
       launch_norm_update(A, lda, jl, m, n, B.work, B.workold, B.st, tol3z, s);
@@ -504,7 +523,7 @@ int run_dm(const Args& x) {
     int q = 0, jend = j0;
     queue_batch(j0, q);
     while (true) {
-      const bool ahead = !x.trace && j0 + batch * KMAX < minmn;
+      const bool ahead = !x.trace && !switch_wanted && j0 + batch * KMAX < minmn;
       if (ahead) queue_batch(j0, q ^ 1);
       DevState* h = wait_slot(q);
       bool in_flight = ahead;
@@ -536,6 +555,7 @@ int run_dm(const Args& x) {
         r.maxnrm = l.maxnrm;
         r.tau = l.tau;
         r.delta = l.delta;
+        r.prec = prec_now;
         for (int t = 0; t < l.fjb && t < 64; t++) r.sel[t] = h->sel[t];
       }
       it = h->it;
@@ -544,12 +564,29 @@ int run_dm(const Args& x) {
         break;
       }
       j0 = h->j;
-      if (in_flight)
+      if (scheduled && prec_now == TRAIL_FP64 &&
+          h->last.maxnrm * sqrt((double)(n - j0)) <= low_bound)
+        switch_wanted = true;
+      if (in_flight) {
         q ^= 1;  // the batch queued ahead is the one to wait for next
-      else
+      } else {
+        if (switch_wanted) {
+          // the pending merged update reaches rows and columns >= j0 (the
+          // R12 rows of the last step are final), then the lower precision
+          if (fused_now && it > 0) {
+            launch_fused(A, lda, j0, m, j0, n, bufs, B.Wp, fused_slabs(j0, m, j0, n), 1, 0,
+                         it & 1, s, nullptr);
+            launch_set_fused(B.st, 0, s);
+          }
+          fused_now = false;
+          bufs = StepBufs{B.V, 0, B.ldv, B.Wf, 0};
+          prec_now = low;
+          switch_wanted = false;
+        }
         queue_batch(j0, q);
+      }
     }
-    if (fused && it > 0) {
+    if (fused_now && it > 0) {
       // the last step's update is still deferred: it reaches the trailing
       // block when the stop test ended the loop before min(m, n)
       const int je = jend;
@@ -821,7 +858,7 @@ int qrdm_dgeqrdm_dev_opt(int matrix_layout, int m, int n, double* d_a, int lda, 
   int bad = check_args(x);
   if (bad) return bad;
   if (opts) {
-    if (opts->trailing < QRDM_TRAILING_FP64 || opts->trailing > QRDM_TRAILING_BF16X3) {
+    if (opts->trailing < QRDM_TRAILING_FP64 || opts->trailing > QRDM_TRAILING_BF16X3_ALL) {
       fprintf(stderr, "DGEQRDM: unknown trailing-update precision %d\n", opts->trailing);
       return -1;
     }

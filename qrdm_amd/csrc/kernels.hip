@@ -1829,6 +1829,14 @@ void launch_resume(DevState* st, hipStream_t s) {
   hipLaunchKernelGGL(k_resume, dim3(1), dim3(1), 0, s, st);
 }
 
+// The trailing update's form from the next step on: 0 = applied within its
+// step (the three-kernel form), after the pending merged update was applied.
+__global__ void k_set_fused(DevState* st, int fused) { st->fused = fused; }
+
+void launch_set_fused(DevState* st, int fused, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_fused, dim3(1), dim3(1), 0, s, st, fused);
+}
+
 
 
 // Swap two whole columns (cblas_dswap of the fixed-column pre-pass,

@@ -306,6 +306,8 @@ void launch_resume(DevState* st, hipStream_t s);
 void launch_swap2(double* A, int lda, int m, int p, int q, hipStream_t s);
 // Zeroes bytes (a multiple of 4) at p with a kernel on s.
 void launch_zero(void* p, size_t bytes, hipStream_t s);
+// Sets DevState::fused (the form of the following steps' trailing update).
+void launch_set_fused(DevState* st, int fused, hipStream_t s);
 
 // ---- the merged trailing update (trailing.hip) -------------------------------
 // Row slabs of a fused pass over rows [r_lo, m), columns [c0, n) (at most
