@@ -31,9 +31,11 @@ extern "C" {
  *          number of columns reduced at DM step it (room for n entries);
  *   thres  thres[0] = delta in [0,1] (cosine bound), thres[1] = tau in [0,1]
  *          (norm threshold), thres[2] = eta for stop mode 3;
- *   nb     maximum DM block (k_max), 1..64 on this device.
+ *   nb     maximum DM block (k_max), 1..256 on this device (steps wider than
+ *          64 columns run host-driven, one synchronisation per step, FP64
+ *          without the qrdm_opts forms).
  * Returns 0 on success; -1 for an illegal argument (a message is printed, as
- * the reference's xerbla_ does); -10 for nb > 64, which this device build
+ * the reference's xerbla_ does); -10 for nb > 256, which this device build
  * does not support (the reference accepts any nb > 0); -13 when the input
  * holds a NaN or an infinity (A is then left untouched);
  * <= -1000 for a device error (see qrdm_last_error).  matrix_layout 101 is
@@ -59,8 +61,8 @@ int qrdm_dgeqrdm_dev(int matrix_layout, int m, int n, double *d_a, int lda, int 
  * harnesses that forbid hipMalloc).  Same arguments, arithmetic and results
  * as qrdm_dgeqrdm_dev; its device workspace is a fixed 64 MiB array inside
  * the library (one per device, held until the process ends), enough for
- * e.g. m = n = 4096 or m = 16384, n = 2048; a larger problem returns -14
- * (message on stderr) before any device work.  No device or pinned memory,
+ * e.g. m = n = 4096 or m = 16384, n = 2048; a larger problem returns -14,
+ * and nb > 64 returns -10 (messages on stderr) before any device work.  No device or pinned memory,
  * events or memset nodes are created: regions that must start at zero are
  * cleared by a kernel, and the DM loop's state is read back into host memory
  * of the library, waiting on the stream.  Calls that overlap on one device

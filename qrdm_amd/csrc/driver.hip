@@ -142,9 +142,16 @@ struct Buffers {
   unsigned* tickets;  // launch tickets (zero between uses): [0] top-k, [1] selection,
                       // [2] reconstruction, [3] T of the Gram-based panel
   int nsplit;
+  // wide steps (nb > KMAX) only: sort keys, the candidate Gram, the
+  // selection's output, the swap pairs
+  double* wval[2] = {nullptr, nullptr};
+  int* widx[2] = {nullptr, nullptr};
+  double* wG = nullptr;  // KW x KW
+  int* wsel = nullptr;   // 2 + KW
+  int* wpairs = nullptr;  // 4 KW
 };
 
-size_t layout_buffers(int m, int n, char* base, Buffers* b) {
+size_t layout_buffers(int m, int n, char* base, Buffers* b, int nb = KMAX) {
   char* p = base;
   const int mpad = ((m + 63) / 64) * 64;
   const int nslab = (m + 31) / 32 + 1;  // slabs of the Gram and fast-panel passes
@@ -174,6 +181,15 @@ size_t layout_buffers(int m, int n, char* base, Buffers* b) {
   t.jp = carve<int>(p, n);
   t.ncols = carve<int>(p, n);
   t.tickets = carve<unsigned>(p, (size_t)(n + 63) / 64 + 4);
+  if (nb > KMAX) {
+    for (int q = 0; q < 2; q++) {
+      t.wval[q] = carve<double>(p, (size_t)wide_sort_keys(n));
+      t.widx[q] = carve<int>(p, (size_t)wide_sort_keys(n));
+    }
+    t.wG = carve<double>(p, (size_t)KW * KW);
+    t.wsel = carve<int>(p, 2 + KW);
+    t.wpairs = carve<int>(p, 4 * KW);
+  }
   if (b) *b = t;
   return (size_t)(p - base);
 }
@@ -243,9 +259,9 @@ int check_args(const Args& x) {
   else if (delta < 0.0 || delta > 1.0) info = -9;
   else if (tau < 0.0 || tau > 1.0) info = -9;
   else if (x.nb <= 0) info = -10;
-  if (info == 0 && x.nb > KMAX) {
+  if (info == 0 && x.nb > KW) {
     fprintf(stderr, "DGEQRDM: nb = %d exceeds the largest DM block of this device build (%d)\n",
-            x.nb, KMAX);
+            x.nb, KW);
     return -10;
   }
   if (info != 0) {
@@ -256,11 +272,11 @@ int check_args(const Args& x) {
   return 0;
 }
 
-Workspace& workspace_for(int m, int n, Buffers* b) {
+Workspace& workspace_for(int m, int n, Buffers* b, int nb = KMAX) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   Workspace& ws = g_ws[dev & 63];
-  size_t need = layout_buffers(m, n, nullptr, nullptr) + 256;
+  size_t need = layout_buffers(m, n, nullptr, nullptr, nb) + 256;
   if (ws.bytes < need) {
     if (ws.base) HIPCHK(hipFree(ws.base));
     ws.base = nullptr;
@@ -273,7 +289,7 @@ Workspace& workspace_for(int m, int n, Buffers* b) {
     HIPCHK(hipHostMalloc(&ws.pinned, ws.pinned_bytes, hipHostMallocDefault));
     for (int q = 0; q < 2; q++) HIPCHK(hipEventCreateWithFlags(&ws.ev[q], hipEventDisableTiming));
   }
-  layout_buffers(m, n, ws.base, b);
+  layout_buffers(m, n, ws.base, b, nb);
   // The buffer offsets depend on (m, n): clear the trailing update's ticket
   // counters, which must start every call at zero.
   HIPCHK(hipMemset(b->tickets, 0, sizeof(unsigned) * ((size_t)(n + 63) / 64 + 4)));
@@ -282,6 +298,102 @@ Workspace& workspace_for(int m, int n, Buffers* b) {
 
 void blocked_qr(double* A, int lda, int m, int n, int ncols_qr, double* tau, Buffers& B,
                 hipStream_t s);
+
+// DM steps wider than KMAX (64 < nb <= KW), host-driven: one synchronisation
+// per step (src/dgeqrdm_work.c:694-787 with k_max up to KW; wide.hip).  The
+// panel is the column-by-column one on all fjb columns; the kept reflectors
+// reach the columns past the block in KMAX-column chunks (H_0 .. H_{k-1} in
+// order, as the reference's single dlarfb applies them); the norm downdate
+// sums the step's fjb_cmp new rows.  Returns 0 (the results are copied back
+// and the stream synchronised).
+int run_wide(const Args& x, Buffers& B, int nfxd, double eta, int enable_stop, double tol3z) {
+  const int m = x.m, n = x.n, lda = x.lda, nb = x.nb;
+  const double delta = x.thres[0], tau_dm = x.thres[1];
+  const int minmn = m < n ? m : n;
+  hipStream_t s = x.s;
+  double* A = x.a;
+  HIPCHK(hipMemcpyAsync(B.jp, x.jpvt, sizeof(int) * n, hipMemcpyHostToDevice, s));
+  std::vector<int> marked(n, 0), hout(2 + KW), hidx(KW), sel, pairs;
+  std::vector<std::pair<int, int>> swaps;
+  int j = nfxd, it = 0;
+  while (j < minmn) {
+    const int msub = m - j, nsub = n - j;
+    const int kmax = std::min(std::min(nb, msub), nsub);
+    // --- DM_perm (src/dgeqrdm_work.c:280-418): candidates, Gram, greedy
+    const double* val = nullptr;
+    launch_topk_wide(B.work + j, nsub, kmax, B.wval, B.widx, &val, s);
+    const int* idx = wide_topk_idx(B.wval, B.widx, val);
+    launch_gram_wide(A, lda, j, m, idx, kmax, B.wG, KW, s);
+    launch_select_wide(B.wG, KW, val, kmax, tau_dm, delta, B.wsel, s);
+    HIPCHK(hipMemcpyAsync(hout.data(), B.wsel, sizeof(int) * (2 + KW), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hidx.data(), idx, sizeof(int) * kmax, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int nc = hout[0], fjb = hout[1];
+    sel.resize(fjb);
+    for (int q = 0; q < fjb; q++) sel[q] = hidx[hout[2 + q]];
+    // permute_marked (:149-262): the walk here, the moves on the device
+    permute_walk(nsub, marked, sel.data(), fjb, swaps);
+    pairs.resize(2 * swaps.size());
+    for (size_t q = 0; q < swaps.size(); q++) {
+      pairs[2 * q] = swaps[q].first;
+      pairs[2 * q + 1] = swaps[q].second;
+    }
+    if (!swaps.empty()) {
+      HIPCHK(hipMemcpyAsync(B.wpairs, pairs.data(), sizeof(int) * pairs.size(),
+                            hipMemcpyHostToDevice, s));
+      launch_swap_pairs(A, lda, m, j, B.work, B.jp, B.wpairs, (int)swaps.size(), s);
+    }
+    // --- dgeqr2_mia (:735-738) on the fjb columns, truncation included
+    launch_panel_exact(A, lda, j, m, fjb, tau_dm, 1, x.tau + j, B.st, s);
+    int kk = 0;
+    HIPCHK(hipMemcpyAsync(&kk, &B.st->fjb_cmp, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // --- dlarft + dlarfb (:740-767) on columns j + fjb .. n, KMAX reflectors
+    // at a time
+    if (j + fjb < n) {
+      for (int i0 = 0; i0 < kk; i0 += KMAX) {
+        const int kq = std::min(KMAX, kk - i0);
+        launch_set_kk(B.st, kq, s);
+        launch_build_v(A, lda, j + i0, m, B.ldv, B.st, B.V, B.ldv, s);
+        launch_gram(B.V, B.ldv, j + i0, m, nullptr, KMAX, nullptr, B.gram_part, B.S, s);
+        launch_larft(B.S, x.tau + j + i0, B.st, B.T, s);
+        launch_larfb(A, lda, j + i0, m, j + fjb, n, B.V, B.ldv, B.T, 1, B.Wp, B.W, B.nsplit, s);
+      }
+    }
+    // --- norm_update (:774-778) over the kk new rows
+    launch_wide_state(B.st, j, kk, s);
+    launch_norm_update(A, lda, j, m, n, B.work, B.workold, B.st, tol3z, s);
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, &B.st->maxnrm_bits, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double maxnrm = 0.0;
+    memcpy(&maxnrm, &bits, sizeof(double));
+    x.ncols[it] = kk;
+    if (x.trace && it < x.ntrace) {
+      qrdm_step& r = x.trace[it];
+      memset(&r, 0, sizeof(r));
+      r.j = j;
+      r.kmax = kmax;
+      r.nc = nc;
+      r.fjb = fjb;
+      r.fjb_cmp = kk;
+      r.nswap = (int)swaps.size();
+      r.panel = 0;
+      r.maxnrm = maxnrm;
+      r.tau = tau_dm;
+      r.delta = delta;
+      r.prec = TRAIL_FP64;
+      for (int q = 0; q < fjb && q < 64; q++) r.sel[q] = sel[q];
+    }
+    it++;
+    j += kk;
+    if (enable_stop && maxnrm * sqrt((double)(nsub - kk)) <= eta) break;  // :782-785
+  }
+  if (x.nsteps) *x.nsteps = it;
+  HIPCHK(hipMemcpyAsync(x.jpvt, B.jp, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
 
 int run_dm(const Args& x) {
   const int m = x.m, n = x.n, lda = x.lda, nb = x.nb;
@@ -308,13 +420,13 @@ int run_dm(const Args& x) {
   hipEvent_t* evs = nullptr;  // completion of each readback slot (none: wait on the stream)
   const bool own_ws = x.scratch == nullptr;
   if (own_ws) {
-    Workspace& ws = workspace_for(m, n, &B);
+    Workspace& ws = workspace_for(m, n, &B, x.nb);
     hst = reinterpret_cast<DevState*>(ws.pinned);
     evs = ws.ev;
   } else {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
-    layout_buffers(m, n, x.scratch, &B);
+    layout_buffers(m, n, x.scratch, &B, x.nb);
     hst = g_host_state[dev & 63];
     // the launch tickets must start at zero (the workspace keeps the last
     // call's contents)
@@ -374,6 +486,7 @@ int run_dm(const Args& x) {
       memcpy(&maxnrm, &bits, sizeof(double));
     }
     eta = eta * maxnrm;
+    if (nb > KMAX) return run_wide(x, B, nfxd, eta, enable_stop, tol3z);
     // --- the DM loop (src/dgeqrdm_work.c:694-787), driven from the device:
     // every kernel of a step reads j and the step's decisions from DevState,
     // so the host queues kBatch steps at a time and synchronises once per
@@ -881,6 +994,13 @@ int qrdm_dgeqrdm_dev_noalloc(int matrix_layout, int m, int n, double* d_a, int l
          (hipStream_t)stream, nullptr, 0, nullptr};
   int bad = check_args(x);
   if (bad) return bad;
+  if (nb > KMAX) {
+    fprintf(stderr,
+            "qrdm_dgeqrdm_dev_noalloc: nb = %d: steps wider than %d are walked by the host with "
+            "lists it allocates; use qrdm_dgeqrdm_dev\n",
+            nb, KMAX);
+    return -10;
+  }
   const size_t need = layout_buffers(m, n, nullptr, nullptr);
   if (need > kScratchBytes) {
     fprintf(stderr,

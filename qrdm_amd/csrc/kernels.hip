@@ -1743,9 +1743,11 @@ __global__ __launch_bounds__(256) void k_norm_update(const double* __restrict__ 
     double wk = work[col];
     if (wk == 0.0) continue;
     const double* a = A + (size_t)col * lda + j;
-    // naive sum of squares of the kk new R rows (src/dgeqrdm_work.c:84-86)
-    const double x = (lane < kk) ? a[lane] : 0.0;
-    const double s = wave_sum(x * x);
+    // sum of squares of the kk new R rows (src/dgeqrdm_work.c:84-86): per
+    // lane rows lane, lane + 64, ... (one term for kk <= 64), then the wave
+    double x2 = 0.0;
+    for (int r = lane; r < kk; r += 64) x2 = fma(a[r], a[r], x2);
+    const double s = wave_sum(x2);
     double temp = sqrt(fabs(s)) / wk;
     const double d2 = (temp + 1.0) * (1.0 - temp);
     temp = (0.0 >= d2) ? 0.0 : d2;

@@ -7,6 +7,8 @@
 namespace qrdm {
 
 constexpr int KMAX = 64;          // largest DM block the kernels handle
+// Widest DM step (nb) a call takes; steps wider than KMAX run host-driven.
+constexpr int KW = 256;
 constexpr int GRAM_ROWS = 64;     // rows per workgroup of the Gram kernel
 
 // Record of one completed DM step (kept for tracing).
@@ -287,6 +289,26 @@ void launch_dm_begin(DevState* st, int j, int minmn, double tau, double delta, i
                      hipStream_t s);
 
 void launch_set_kk(DevState* st, int kk, hipStream_t s);
+// ---- wide DM steps, 64 < nb <= KW (wide.hip): host-driven, one sync per step.
+// The top kmax (<= KW) of work[0..nsub) (value descending, index ascending)
+// by sort passes through the two key buffers (wide_sort_keys(n) entries
+// each); *out_val gets the buffer holding them, wide_topk_idx its indices.
+int wide_sort_keys(int n);
+void launch_topk_wide(const double* work, int nsub, int kmax, double* val[2], int* idx[2],
+                      const double** out_val, hipStream_t s);
+const int* wide_topk_idx(double* val[2], int* idx[2], const double* out_val);
+// Gram of the nc candidate columns j + idx[c] over rows [j, m) (tile pairs
+// on or above the diagonal of 64 x 64 tiles; G column-major, ld ldg).
+void launch_gram_wide(const double* A, int lda, int j, int m, const int* idx, int nc, double* G,
+                      int ldg, hipStream_t s);
+// out[0] = nc, out[1] = fjb, out[2 + s] = rank of the s-th accepted candidate.
+void launch_select_wide(const double* G, int ldg, const double* val, int kmax, double tau,
+                        double delta, int* out, hipStream_t s);
+// np position swaps (pairs relative to column j) of A's columns, work, jpvt.
+void launch_swap_pairs(double* A, int lda, int m, int j, double* work, int* jp, const int* pairs,
+                       int np, hipStream_t s);
+// st->j = j, st->fjb_cmp = kk, the running max cleared.
+void launch_wide_state(DevState* st, int j, int kk, hipStream_t s);
 // Tall-panel DM QR without per-column grid reductions (panel.hip), RUN_FAST
 // kernels: j, fjb from st.  launch_perm_chol1 moves the step's columns (as
 // launch_apply_perm) and factors the panel's Gram, gathered from the candidate

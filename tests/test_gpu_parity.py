@@ -336,8 +336,8 @@ def test_nonfinite_input_returns_minus13(gpu):
     assert run_host(a)["info"] == -13
 
 
-def test_nb_above_64_is_rejected(gpu):
-    assert run_host(inputs.gaussian(80, 80, 1), nb=65)["info"] == -10
+def test_nb_above_256_is_rejected(gpu):
+    assert run_host(inputs.gaussian(80, 80, 1), nb=257)["info"] == -10
 
 
 # ----------------------------------------------- QRF / DORMQR / Python module
