@@ -403,12 +403,13 @@ def trailing_forms(torch, qrdm_amd, args, timed, work, pristine, tau, last_jp, s
 def batched_form(torch, qrdm_amd, steps, loop_count, autotune=True):
     """4096 uneven matrices (m in [64, 1024], n in [32, min(m, 512)], spectra
     log-uniform over [1e-10, 1]) in one persistent launch (4 waves per
-    workgroup, nb 64, replayed from a hipGraph), against a loop of the plain
-    path."""
+    workgroup, nb 64, the panel form autotuned, replayed from a hipGraph),
+    beside the column-by-column panel form and a loop of the plain path."""
     shapes = batch_shapes(4096, 4)
     plan = qrdm_amd.BatchPlan(shapes, nb=64)
     src, _ = batch_fill(torch, plan, 4)
     q = plan.autotune(src) if autotune else plan.query()
+    rows = q["panel_rows"]
     work = torch.empty_like(src)
     tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
     jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
@@ -426,6 +427,17 @@ def batched_form(torch, qrdm_amd, steps, loop_count, autotune=True):
     ms = 1000.0 * total / steps
     if int((info != 0).sum().item()) != 0:
         raise RuntimeError("batched: non-zero info")
+    # the column-by-column panel form, for reference
+    plan.set_panel_rows(0)
+    ms_cbc = []
+    for rep in range(2):
+        work.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.run(work, tau, jp, nc, info, graph=True)
+        torch.cuda.synchronize()
+        ms_cbc.append(1000.0 * (time.perf_counter() - t0))
+    plan.set_panel_rows(rows)
     import numpy as np
     work.copy_(src)
     tau2 = torch.zeros_like(tau)
@@ -446,6 +458,7 @@ def batched_form(torch, qrdm_amd, steps, loop_count, autotune=True):
     return {"workload": "4096 uneven FP64 matrices, m in [64,1024], n in [32,min(m,512)], "
                         "U diag(s) V^T with s log-uniform over [1e-10,1], nb=64",
             "ms_per_batch": round(ms, 2), "tfs": round(fl / (ms / 1000.0) / 1e12, 4),
+            "ms_column_by_column_panel": round(ms_cbc[-1], 2),
             "flops": fl, "per_matrix_loop_matrices": min(loop_count, len(shapes)),
             "per_matrix_loop_ms": round(loop_ms, 1),
             "speedup_vs_loop": (round(loop_ms / ms, 2) if loop_count >= len(shapes) else None),
@@ -493,7 +506,7 @@ def run_form_child(key, batch_loop):
     import qrdm_amd
     torch.cuda.set_device(0)
     if key == "batched":
-        out = batched_form(torch, qrdm_amd, 3, batch_loop, autotune=False)
+        out = batched_form(torch, qrdm_amd, 3, batch_loop, autotune=True)
     elif key == "learned":
         out = learned_form(torch)
     else:
@@ -513,14 +526,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 1024x512, 16384^2 and host-pointer runs")
-    ap.add_argument("--forms", default="fp32,bf16x3,fp32_all,learned",
+    ap.add_argument("--forms", default="fp32,bf16x3,fp32_all,batched,learned",
                     help="comma list of the added forms to measure beside the plain path "
                          "(fp32, bf16x3: the scheduled reduced-precision trailing updates; "
                          "fp32_all, bf16x3_all: the reduced precision in every step; "
                          "batched, learned; empty for none).  batched and "
-                         "learned run in child processes; batched is not in the default list: "
-                         "its child ended in a host-heap abort whose cause is not identified "
-                         "(the batched form is measured by tests/test_gpu_batched.py)")
+                         "learned run in child processes (a failure there is reported in the "
+                         "line)")
     ap.add_argument("--form-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--batch-loop", type=int, default=4096,
                     help="matrices in the per-matrix loop the batched form is compared with")

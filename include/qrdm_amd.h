@@ -182,7 +182,8 @@ typedef struct qrdm_batch_plan qrdm_batch_plan;
  * Returns NULL on error (text through qrdm_last_error). */
 qrdm_batch_plan *qrdm_batch_plan_create(int count, const qrdm_batch_item *items, int nb);
 void qrdm_batch_plan_destroy(qrdm_batch_plan *plan);
-/* out[0..4) = {waves per workgroup, workgroups, nb, timed candidates}. */
+/* out[0..5) = {waves per workgroup, workgroups, nb, timed candidates,
+ * panel rows (see qrdm_batch_plan_set_panel_rows)}; returns 5. */
 int qrdm_batch_plan_query(const qrdm_batch_plan *plan, int *out, int nout);
 /* Uses a learned (tau, delta) network (qrdm_policy in device memory, or
  * NULL for thres) in every factorisation of the batch. */
@@ -201,10 +202,20 @@ int qrdm_batch_trace(qrdm_batch_plan *plan, double *d_a, double *d_tau, int *d_j
 /* Sets the launch shape by hand: waves per workgroup, 4 (the one shape
  * built; 8 is switched off and returns -1). */
 int qrdm_batch_plan_set_shape(qrdm_batch_plan *plan, int waves);
+/* The panel form by hand: a DM step whose panel has at least `rows` rows
+ * (and two or more columns) takes the Gram-based form -- Cholesky of the
+ * panel's Gram, one re-orthogonalisation (CholeskyQR2), Householder vectors
+ * reconstructed from the Q factor, T from V^T V -- the others the
+ * column-by-column dgeqr2 form; rows <= 0: column by column only.  A new
+ * plan starts at 128 rows.  Both forms meet the same parity bar; they round
+ * differently, so the choice can change pivots decided by rounding.  No
+ * counterpart in the reference. */
+int qrdm_batch_plan_set_panel_rows(qrdm_batch_plan *plan, int rows);
 /* Times the candidate launches (nb in {16, 32, 64} when the plan was
- * created with nb <= 0, else the plan's nb; 4 waves per workgroup) on a
- * scratch copy of the batch d_a (left unchanged) and keeps the fastest.
- * thres = {delta, tau}.  Synchronous. */
+ * created with nb <= 0, else the plan's nb; times panel rows in {0 (column
+ * by column), 128, 256}; 4 waves per workgroup) on a scratch copy of the
+ * batch d_a (left unchanged) and keeps the fastest.  thres = {delta, tau}.
+ * Synchronous. */
 int qrdm_batch_autotune(qrdm_batch_plan *plan, const double *d_a, const double *thres,
                         void *stream);
 /* Factorises the whole batch in place: one kernel launch on stream,

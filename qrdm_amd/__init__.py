@@ -48,7 +48,8 @@ EXPORTS = ("dgeqrdm", "qrdm_dgeqrdm_dev", "qrdm_dgeqrdm_dev_noalloc", "qrdm_dgeq
            "qrdm_dormqr_dev", "qrdm_dgeqrf", "qrdm_dormqr", "qrdm_permute_walk",
            "qrdm_profile_enable", "qrdm_profile_select", "qrdm_profile_reset",
            "qrdm_profile_read", "qrdm_batch_plan_create", "qrdm_batch_plan_destroy",
-           "qrdm_batch_plan_query", "qrdm_batch_plan_set_shape", "qrdm_batch_autotune",
+           "qrdm_batch_plan_query", "qrdm_batch_plan_set_shape",
+           "qrdm_batch_plan_set_panel_rows", "qrdm_batch_autotune",
            "qrdm_batch_plan_set_policy", "qrdm_batch_trace", "qrdm_policy_sizes",
            "qrdm_policy_grad", "qrdm_policy_adam", "qrdm_policy_pack", "qrdm_policy_predict", "qrdm_dgeqrdm_batched",
            "qrdm_last_error", "qrdm_version")
@@ -83,6 +84,7 @@ def lib():
         L.qrdm_batch_plan_destroy.argtypes = [V]
         L.qrdm_batch_plan_query.argtypes = [V, _ip, ctypes.c_int]
         L.qrdm_batch_plan_set_shape.argtypes = [V, ctypes.c_int]
+        L.qrdm_batch_plan_set_panel_rows.argtypes = [V, ctypes.c_int]
         L.qrdm_batch_plan_set_policy.argtypes = [V, V]
         _fp = ctypes.POINTER(ctypes.c_float)
         L.qrdm_batch_trace.argtypes = [V, V, V, V, V, V, _dp, V, ctypes.c_int, V, _fp, V]
@@ -205,9 +207,17 @@ class BatchPlan:
 
     def query(self):
         import numpy as np
-        out = np.zeros(4, np.int32)
-        lib().qrdm_batch_plan_query(self._p, out.ctypes.data_as(_ip), 4)
-        return dict(waves=int(out[0]), grid=int(out[1]), nb=int(out[2]), tuned=int(out[3]))
+        out = np.zeros(5, np.int32)
+        lib().qrdm_batch_plan_query(self._p, out.ctypes.data_as(_ip), 5)
+        return dict(waves=int(out[0]), grid=int(out[1]), nb=int(out[2]), tuned=int(out[3]),
+                    panel_rows=int(out[4]))
+
+    def set_panel_rows(self, rows):
+        """Panels with at least rows rows (and 2+ columns) take the Gram-based
+        form, the others the column-by-column one; rows <= 0: column by
+        column only.  A new plan starts at 128."""
+        if lib().qrdm_batch_plan_set_panel_rows(self._p, rows) != 0:
+            raise ValueError(last_error())
 
     def set_shape(self, waves):
         """Launch shape by hand: 4 waves per workgroup (the one shape built;
@@ -217,7 +227,8 @@ class BatchPlan:
 
     def autotune(self, a, thres=(0.9, 0.15), stream=None):
         """Times the candidate launches (nb in {16, 32, 64} for a plan made
-        with nb <= 0) on a scratch copy of the batch tensor a."""
+        with nb <= 0; panel rows in {0, 128, 256}) on a scratch copy of the
+        batch tensor a and keeps the fastest."""
         import numpy as np
         th = np.ascontiguousarray(list(thres) + [0.0] * (3 - len(thres)), dtype=np.float64)
         info = lib().qrdm_batch_autotune(self._p, ctypes.c_void_p(a.data_ptr()),

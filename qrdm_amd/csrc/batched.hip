@@ -30,19 +30,27 @@ struct BatchParams {
   unsigned* trace_count;    // records written (may exceed trace_cap)
   int trace_cap;
   float tau_lo, tau_hi, delta_lo, delta_hi;  // label normalisation
+  int gram_rows;  // panels of at least this many rows (and 2+ columns) take the Gram-based form
 };
 
 template <int NW>
 struct BatchLds {
   double work[BT_NMAX], wold[BT_NMAX];
   int jp[BT_NMAX];
-  double lv[NW][KMAX];
-  int li[NW][KMAX];
-  double R1[KMAX * (KMAX + 1)];  // Gram slab / G / V^T V / trailing-update W
-  double Ts[KMAX * KMAX];
+  union {
+    struct {
+      double lv[NW][KMAX];
+      int li[NW][KMAX];
+    } tk;                       // the top-k's per-wave lists
+    double E[2][KMAX][LDP];     // the Gram-based panel's matrices
+  } u;
+  double R1[KMAX * LDP];  // Gram slab / G / V^T V / trailing-update W
+  double Ts[KMAX * LDP];  // T (column-major, leading dimension KMAX)
   double ts[KMAX], tmp[KMAX];
+  double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
   double red[16];
   int perm[4 * KMAX + KMAX];
+  int s_done, s_why;
   DevState st;
   float feat[POL_IN];
   float hbuf[POL_H];
@@ -65,10 +73,10 @@ __device__ __forceinline__ double bt_v(const double* a, int lda, int j, int kk, 
 // LDS.  W = T^T P (waves 0..3, rows 16w..), then C -= V W as C^T -= W^T V^T,
 // wave w taking the 16-row blocks w, w + NW, ...
 template <int NW>
-__device__ void bt_trailing(double* a, int lda, int j, int m, int kk, int c0, int n,
+__device__ __forceinline__ void bt_trailing(double* a, int lda, int j, int m, int kk, int c0, int n,
                             BatchLds<NW>& L) {
   constexpr int UPW = 16 / NW;  // 16-column groups per wave in P
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int wr = wave & 3, u0 = UPW * (wave >> 2);
   double* Wl = L.R1;  // [k][c], leading dimension KMAX + 1
@@ -191,10 +199,10 @@ __device__ void bt_trailing(double* a, int lda, int j, int m, int kk, int c0, in
 // into R1 (leading dimension KMAX + 1); waves 0..3 run the matrix cores,
 // every wave loads.
 template <int NW, typename ColFn>
-__device__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L) {
+__device__ __forceinline__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L) {
   constexpr int NT = 64 * NW;
   double* X = L.R1;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = local_tid(), wave = tid >> 6;
   dbl4 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
@@ -226,13 +234,134 @@ __device__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L)
   __syncthreads();
 }
 
+// S += the 64 x 64 product mfma_64 left in acc (each wave adds its own rows).
+__device__ __forceinline__ void bt_add_tile(double (*S)[LDP], const dbl4p (&acc)[4]) {
+  const int t = local_tid(), lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) S[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] += acc[u][v];
+}
+
+// The Gram-based panel (panel.hip's tall-panel form) by the workgroup: R1 =
+// chol of the panel's Gram, gathered from the candidate Gram in L.R1 through
+// the ranks select_wave left in perm[4 KMAX ..] (chol1_core); Q = P R1^-1
+// and G2 = Q^T Q over 64-row slabs; the reconstruction (recon_core: R, V1,
+// tau, M); V2 = P_bottom M written into the panel, with V^T V; then tau and
+// T from V^T V (the compact WY identity, as k_reduce_t), T into L.Ts.
+// Returns the columns kept, or -1 with A and tau unchanged when the form
+// falls back (the caller then runs the column-by-column panel).
+template <int NW>
+__device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, int fjb, double tau_dm,
+                             double* tau, BatchLds<NW>& L) {
+  static_assert(NW == 4, "the one-workgroup factorisations run on four waves");
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6;
+  double (*E0)[LDP] = L.u.E[0];
+  double (*E1)[LDP] = L.u.E[1];
+  double (*RT)[LDP] = reinterpret_cast<double (*)[LDP]>(L.R1);
+  double (*TS)[LDP] = reinterpret_cast<double (*)[LDP]>(L.Ts);
+  auto Yof = [](double* p) { return reinterpret_cast<double (*)[BLK][BLK + 1]>(p); };
+  // 1. R1 and R1^-1 (E0, E1; zero outside k1 x k1)
+  const int* rank = L.perm + 4 * KMAX;
+  if (tid < KMAX) {
+    const int rk = tid < fjb ? rank[tid] : 0;
+    L.tmp[tid] = tid < fjb ? L.R1[rk * (KMAX + 2)] : 0.0;  // G(rk, rk)
+  }
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) {
+    const int r = e >> 6, c = e & 63;
+    E0[r][c] = (r < fjb && c < fjb) ? L.R1[rank[r] + rank[c] * (KMAX + 1)] : (r == c ? 1.0 : 0.0);
+  }
+  lds_barrier();
+  const int k1 = chol1_core(E0, E1, Yof(L.Ts), L.rinv, L.tmp, fjb, tau_dm, &L.s_done, &L.s_why);
+  if (k1 < 0) return -1;
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) {
+    const int r = e >> 6, c = e & 63;
+    if (!(r < k1 && c < k1)) {
+      E0[r][c] = 0.0;
+      E1[r][c] = 0.0;
+    }
+  }
+  lds_barrier();
+  // 2. G2 = (P R1^-1)^T (P R1^-1), slab by slab through TS, summed into RT
+  // (each wave adds its own rows); identity beyond k1
+  dbl4p acc[4];
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) RT[e >> 6][e & 63] = 0.0;
+  for (int rs = j; rs < m; rs += PR_ROWS) {
+    load_slab(TS, a, lda, j, rs, m, k1);
+    lds_barrier();
+    mfma_64(acc, TS, false, E1);
+    lds_barrier();
+    mfma_store(TS, acc);
+    lds_barrier();
+    mfma_64(acc, TS, true, TS);
+    bt_add_tile(RT, acc);
+    lds_barrier();  // the next slab overwrites TS
+  }
+  if (tid >= k1 && tid < KMAX) RT[tid][tid] = 1.0;
+  lds_barrier();
+  // 3. the reconstruction: SR = E0, SA = E1 (R1^-1), SW = TS (also its
+  // scratch Y, and M out), SX = RT
+  double* Mf = &TS[0][0];  // M(q, c) at Mf[q + c * KMAX]
+  const int k = recon_core(E0, E1, TS, RT, Yof(L.Ts), L.rinv, L.dsg, L.tl, L.scv, &L.s_done,
+                           &L.s_why, nullptr, a, lda, j, fjb, k1, tau_dm, tau + j, Mf);
+  if (k < 0) return -1;
+  __syncthreads();  // the top block the reconstruction wrote to A is read below
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) E0[e & 63][e >> 6] = Mf[e];  // M, padded
+  lds_barrier();
+  // 4. V2 = P_bottom M, slab by slab through RT, written into the panel below
+  // the top block (which keeps V1); V^T V summed into E1
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) E1[e >> 6][e & 63] = 0.0;
+  for (int rs = j; rs < m; rs += PR_ROWS) {
+    load_slab(RT, a, lda, j, rs, m, k);
+    lds_barrier();
+    mfma_64(acc, RT, false, E0);
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
+        const int pr = rs - j + r;
+        double x = acc[u][v];
+        if (c >= k) x = 0.0;
+        else if (pr < k) x = (pr < c) ? 0.0 : (pr == c ? 1.0 : RT[r][c]);
+        acc[u][v] = x;
+      }
+    lds_barrier();
+    mfma_store(RT, acc);
+    lds_barrier();
+    for (int e = tid; e < KMAX * PR_ROWS; e += NW * 64) {
+      const int c = e / PR_ROWS, r = e % PR_ROWS, row = rs + r;
+      if (row < m && c < k && row - j >= k) a[(size_t)(j + c) * lda + row] = RT[r][c];
+    }
+    mfma_64(acc, RT, true, RT);
+    bt_add_tile(E1, acc);
+    lds_barrier();  // the next slab overwrites RT
+  }
+  // 5. tau_i = 2 / S_ii and T = (striu(S) + diag(S) / 2)^-1 from S = V^T V
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) {
+    const int r = e & 63, c = e >> 6;
+    const double sv = E1[r][c];
+    E0[r][c] = (r < k && c < k) ? ((r < c) ? sv : (r == c ? 0.5 * sv : 0.0)) : (r == c ? 1.0 : 0.0);
+  }
+  lds_barrier();
+  if (tid < k) tau[j + tid] = 1.0 / E0[tid][tid];
+  lu_inverse<false>(E0, E1, L.rinv, Yof(L.R1));  // E1 = T (upper)
+  for (int e = tid; e < KMAX * KMAX; e += NW * 64) {
+    const int r = e & 63, c = e >> 6;
+    L.Ts[r + c * KMAX] = (r < k && c < k && r <= c) ? E1[r][c] : 0.0;
+  }
+  __syncthreads();  // V2 in A and T in LDS, for the trailing update
+  return k;
+}
+
 // One DM-QR (dgeqrdm semantics, free columns) of the matrix it by the whole
 // workgroup.  Returns info (0, or -13 for a non-finite entry, A untouched).
 template <int NW>
-__device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* jp_all,
+__device__ __forceinline__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* jp_all,
                          int* nc_all, const BatchParams& P, BatchLds<NW>& L) {
   constexpr int NT = 64 * NW;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
   const int m = it.m, n = it.n, lda = it.lda;
   double* a = A + it.a_off;
   double* tau = tau_all + it.tau_off;
@@ -259,6 +388,9 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
   eta *= maxnrm0;
   int j = 0, itn = 0;
   while (j < minmn) {
+    // the workitem id afresh for each step: what the step derives from it is
+    // not held in registers across the steps
+    const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
     const int nsub = n - j;
     int kmax = P.nb < KMAX ? P.nb : KMAX;
     if (m - j < kmax) kmax = m - j;
@@ -275,16 +407,16 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
         wave_sort64(bv, bi);
         wave_merge64(v, ix, bv, bi);
       }
-      L.lv[wave][lane] = v;
-      L.li[wave][lane] = ix;
+      L.u.tk.lv[wave][lane] = v;
+      L.u.tk.li[wave][lane] = ix;
       __syncthreads();
-      block_merge_lists<NW>(L.lv, L.li, v, ix);
+      block_merge_lists<NW>(L.u.tk.lv, L.u.tk.li, v, ix);
       if (wave == 0) {
-        const double cv = L.lv[0][lane];
-        const int ci = L.li[0][lane];
+        const double cv = L.u.tk.lv[0][lane];
+        const int ci = L.u.tk.li[0][lane];
         L.st.cand_val[lane] = lane < kmax ? cv : 0.0;
         L.st.cand_idx[lane] = lane < kmax ? ci : -1;
-        const double lim = P.tau_dm * L.lv[0][0];
+        const double lim = P.tau_dm * L.u.tk.lv[0][0];
         const int nc = __popcll(__ballot(lane < kmax && cv > lim));
         if (lane == 0) {
           L.st.nc = nc;
@@ -302,7 +434,7 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
       if (P.policy || P.trace)
         policy_features(L.work + j, nsub, L.st.cand_val[0], j, m, n, L.feat, L.hist);
       if (P.policy && wave == 0) {
-        const double cv = L.lv[0][lane];
+        const double cv = L.u.tk.lv[0][lane];
         const float2 y = policy_forward(P.policy, L.feat, L.hbuf);
         double tp = P.policy->tau_lo + (P.policy->tau_hi - P.policy->tau_lo) * (double)y.x;
         double dp = P.policy->delta_lo + (P.policy->delta_hi - P.policy->delta_lo) * (double)y.y;
@@ -310,7 +442,7 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
           tp = P.tau_dm;
           dp = P.delta;
         }
-        const int nc2 = __popcll(__ballot(lane < kmax && cv > tp * L.lv[0][0]));
+        const int nc2 = __popcll(__ballot(lane < kmax && cv > tp * L.u.tk.lv[0][0]));
         if (lane == 0) {
           L.st.nc = nc2;
           L.st.policy_on = 1;
@@ -374,32 +506,44 @@ __device__ int bt_factor(const BatchItem& it, double* A, double* tau_all, int* j
       }
       __syncthreads();
     }
-    // --- dgeqr2_mia with truncation (src/dgeqrdm_work.c:735-738)
-    const int kk = panel_exact_body<4>(a, lda, j, m, fjb, P.tau_dm, 1, tau + j, L.red);
-    __syncthreads();
-    // --- dlarft from S = V^T V (src/dgeqrdm_work.c:751-754)
-    const int c0 = j + fjb;
+    // --- dgeqr2_mia with truncation (src/dgeqrdm_work.c:735-738): the
+    // Gram-based form on tall panels (T from its V^T V; the trailing update
+    // also reaches the rejected in-block columns, as dlarf_ did inside the
+    // reference's panel), else, or when it falls back, column by column
+    int kk = -1, c0 = 0;
+    if (fjb >= 2 && m - j >= P.gram_rows) {
+      kk = bt_panel_gram<NW>(a, lda, j, m, fjb, P.tau_dm, tau, L);
+      c0 = j + kk;
+    }
+    if (kk < 0) {
+      kk = panel_exact_body<4>(a, lda, j, m, fjb, P.tau_dm, 1, tau + j, L.red);
+      __syncthreads();
+      c0 = j + fjb;
+      // --- dlarft from S = V^T V (src/dgeqrdm_work.c:751-754)
+      if (c0 < n) {
+        bt_gram<NW>(j, m, kk, [&](int c, int r) { return bt_v(a, lda, j, kk, r, c, m); }, L);
+        // R1 holds S with leading dimension KMAX + 1: repack to KMAX for the recurrence
+        double sv[KMAX * KMAX / NT];
+#pragma unroll
+        for (int q = 0; q < KMAX * KMAX / NT; q++) {
+          const int e = tid + NT * q;
+          sv[q] = L.R1[(e & 63) + (e >> 6) * (KMAX + 1)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < KMAX * KMAX / NT; q++) {
+          const int e = tid + NT * q;
+          L.R1[e] = sv[q];
+          L.Ts[e] = 0.0;
+        }
+        if (tid < KMAX) L.ts[tid] = tid < kk ? tau[j + tid] : 0.0;
+        __syncthreads();
+        larft_recurrence(L.R1, L.Ts, L.ts, L.tmp, kk);
+        __syncthreads();
+      }
+    }
+    // --- dlarfb on columns c0..n (src/dgeqrdm_work.c:762-767)
     if (c0 < n) {
-      bt_gram<NW>(j, m, kk, [&](int c, int r) { return bt_v(a, lda, j, kk, r, c, m); }, L);
-      // R1 holds S with leading dimension KMAX + 1: repack to KMAX for the recurrence
-      double sv[KMAX * KMAX / NT];
-#pragma unroll
-      for (int q = 0; q < KMAX * KMAX / NT; q++) {
-        const int e = tid + NT * q;
-        sv[q] = L.R1[(e & 63) + (e >> 6) * (KMAX + 1)];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < KMAX * KMAX / NT; q++) {
-        const int e = tid + NT * q;
-        L.R1[e] = sv[q];
-        L.Ts[e] = 0.0;
-      }
-      if (tid < KMAX) L.ts[tid] = tid < kk ? tau[j + tid] : 0.0;
-      __syncthreads();
-      larft_recurrence(L.R1, L.Ts, L.ts, L.tmp, kk);
-      __syncthreads();
-      // --- dlarfb on columns j+fjb..n (src/dgeqrdm_work.c:762-767)
       bt_trailing<NW>(a, lda, j, m, kk, c0, n, L);
       __syncthreads();
     }
@@ -476,11 +620,12 @@ int launch_dm_batch(int waves, int grid, const void* items, const int* order, in
                      int* queue, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
                      double tol3z, const void* policy, float* trace, unsigned* trace_count,
-                     int trace_cap, const float* ranges, hipStream_t s) {
+                     int trace_cap, const float* ranges, int gram_rows, hipStream_t s) {
   BatchParams P{delta, tau_dm, eta3, tol3z, nb, stop_mode,
                 static_cast<const PolicyNet*>(policy), trace, trace_count, trace_cap,
                 ranges ? ranges[0] : 0.f, ranges ? ranges[1] : 1.f,
-                ranges ? ranges[2] : 0.f, ranges ? ranges[3] : 1.f};
+                ranges ? ranges[2] : 0.f, ranges ? ranges[3] : 1.f,
+                gram_rows > 0 ? gram_rows : INT_MAX};
   const BatchItem* it = static_cast<const BatchItem*>(items);
   // Only the 4-wave shape is instantiated: runs of k_dm_batch<8> on a
   // 4096-matrix batch ended in host-heap aborts of unidentified cause.

@@ -828,6 +828,7 @@ struct qrdm_batch_plan {
   int nb_req = 0;   // the caller's nb (<= 0: picked by qrdm_batch_autotune)
   int nb_run = 64;  // nb of the runs
   int waves = 4;    // 64-thread waves per workgroup (4 or 8)
+  int gram_rows = 128;  // panels of >= this many rows take the Gram-based form (0: none)
   int grid = 0;
   std::vector<qrdm_batch_item> items;
   void* d_items = nullptr;
@@ -840,14 +841,14 @@ struct qrdm_batch_plan {
   struct Key {
     const void *a, *tau, *jp, *nc, *info;
     double th[3];
-    int stop, waves, nb;
+    int stop, waves, nb, gram_rows;
     bool operator==(const Key& o) const {
       return a == o.a && tau == o.tau && jp == o.jp && nc == o.nc && info == o.info &&
              th[0] == o.th[0] && th[1] == o.th[1] && th[2] == o.th[2] && stop == o.stop &&
-             waves == o.waves && nb == o.nb;
+             waves == o.waves && nb == o.nb && gram_rows == o.gram_rows;
     }
   } key{};
-  double tuned_ms[8] = {};
+  double tuned_ms[16] = {};
   int tuned_n = 0;
   const void* policy = nullptr;  // learned (tau, delta) network on the device, or null
 };
@@ -880,13 +881,13 @@ void batch_release_graph(qrdm_batch_plan* p) {
   p->graph = nullptr;
 }
 
-void batch_launch(qrdm_batch_plan* p, int waves, int nb, double* a, double* tau, int* jp,
-                  int* nc, int* info, const double* th, int stop, hipStream_t s,
+void batch_launch(qrdm_batch_plan* p, int waves, int nb, int gram_rows, double* a, double* tau,
+                  int* jp, int* nc, int* info, const double* th, int stop, hipStream_t s,
                   float* trace = nullptr, unsigned* trace_count = nullptr, int trace_cap = 0,
                   const float* ranges = nullptr) {
   if (launch_dm_batch(waves, p->grid, p->d_items, p->d_sched, p->count, p->d_queue, a, tau, jp,
                       nc, info, nb, th[0], th[1], stop, th[2], sqrt(kEps), p->policy, trace,
-                      trace_count, trace_cap, ranges, s) != 0)
+                      trace_count, trace_cap, ranges, gram_rows, s) != 0)
     throw HipError{hipErrorInvalidValue, "batch launch shape"};
   HIPCHK(hipGetLastError());
 }
@@ -1182,10 +1183,10 @@ void qrdm_batch_plan_destroy(qrdm_batch_plan* p) {
 
 int qrdm_batch_plan_query(const qrdm_batch_plan* p, int* out, int nout) {
   if (!p || !out) return -1;
-  const double vals[4] = {(double)p->waves, (double)p->grid, (double)p->nb_run,
-                          (double)p->tuned_n};
-  for (int i = 0; i < nout && i < 4; i++) out[i] = (int)vals[i];
-  return 4;
+  const double vals[5] = {(double)p->waves, (double)p->grid, (double)p->nb_run,
+                          (double)p->tuned_n, (double)p->gram_rows};
+  for (int i = 0; i < nout && i < 5; i++) out[i] = (int)vals[i];
+  return 5;
 }
 
 int qrdm_batch_plan_set_policy(qrdm_batch_plan* p, const void* d_policy) {
@@ -1204,8 +1205,8 @@ int qrdm_batch_trace(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_jpvt
   return guarded_call([&] {
     hipStream_t s = (hipStream_t)stream;
     const double th[3] = {thres[0], thres[1], 0.0};
-    batch_launch(p, p->waves, p->nb_run, d_a, d_tau, d_jpvt, d_ncols, d_info, th, 0, s,
-                 d_records, d_count, cap, ranges);
+    batch_launch(p, p->waves, p->nb_run, p->gram_rows, d_a, d_tau, d_jpvt, d_ncols, d_info, th, 0,
+                 s, d_records, d_count, cap, ranges);
     return 0;
   });
 }
@@ -1220,6 +1221,13 @@ int qrdm_batch_plan_set_shape(qrdm_batch_plan* p, int waves) {
     return -1;
   }
   p->waves = waves;
+  batch_release_graph(p);
+  return 0;
+}
+
+int qrdm_batch_plan_set_panel_rows(qrdm_batch_plan* p, int rows) {
+  if (!p) return -1;
+  p->gram_rows = rows > 0 ? rows : 0;
   batch_release_graph(p);
   return 0;
 }
@@ -1250,33 +1258,36 @@ int qrdm_batch_autotune(qrdm_batch_plan* p, const double* d_a, const double* thr
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
       const int nbs_auto[3] = {16, 32, 64};
+      const int rows_c[3] = {0, 128, 256};  // the panel form: column by column, or Gram-based
       const int nnb = p->nb_req > 0 ? 1 : 3;
       float best = 1e30f;
-      int bw = p->waves, bnb = p->nb_run;
+      int bw = p->waves, bnb = p->nb_run, brows = p->gram_rows;
       p->tuned_n = 0;
       for (int ib = 0; ib < nnb; ib++) {
         const int nb = p->nb_req > 0 ? p->nb_req : nbs_auto[ib];
-        {
+        for (int ir = 0; ir < 3; ir++) {
           const int waves = 4;  // the one launch shape (8 waves is switched off)
           float t = 0.f;
           for (int rep = 0; rep < 2; rep++) {  // the first run warms up
             HIPCHK(hipMemcpyAsync(a, d_a, sizeof(double) * p->a_ext, hipMemcpyDeviceToDevice, s));
             HIPCHK(hipEventRecord(e0, s));
-            batch_launch(p, waves, nb, a, tau, jp, nc, info, thres, 0, s);
+            batch_launch(p, waves, nb, rows_c[ir], a, tau, jp, nc, info, thres, 0, s);
             HIPCHK(hipEventRecord(e1, s));
             HIPCHK(hipEventSynchronize(e1));
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
           }
-          if (p->tuned_n < 8) p->tuned_ms[p->tuned_n++] = t;
+          if (p->tuned_n < 16) p->tuned_ms[p->tuned_n++] = t;
           if (t < best) {
             best = t;
             bw = waves;
             bnb = nb;
+            brows = rows_c[ir];
           }
         }
       }
       p->waves = bw;
       p->nb_run = bnb;
+      p->gram_rows = brows;
       batch_release_graph(p);
     } catch (...) {
       cleanup();
@@ -1298,11 +1309,12 @@ int qrdm_dgeqrdm_batched(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_
     hipStream_t s = (hipStream_t)stream;
     const double th[3] = {thres[0], thres[1], stop_mode == 3 ? thres[2] : 0.0};
     if (!use_graph) {
-      batch_launch(p, p->waves, p->nb_run, d_a, d_tau, d_jpvt, d_ncols, d_info, th, stop_mode, s);
+      batch_launch(p, p->waves, p->nb_run, p->gram_rows, d_a, d_tau, d_jpvt, d_ncols, d_info, th,
+                   stop_mode, s);
       return 0;
     }
     qrdm_batch_plan::Key k{d_a, d_tau, d_jpvt, d_ncols, d_info, {th[0], th[1], th[2]},
-                           stop_mode, p->waves, p->nb_run};
+                           stop_mode, p->waves, p->nb_run, p->gram_rows};
     if (!p->exec || !(k == p->key)) {
       batch_release_graph(p);
       HIPCHK(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
@@ -1310,7 +1322,7 @@ int qrdm_dgeqrdm_batched(qrdm_batch_plan* p, double* d_a, double* d_tau, int* d_
                                      p->d_queue, d_a, d_tau, d_jpvt, d_ncols, d_info, p->nb_run,
                                      th[0], th[1],
                                      stop_mode, th[2], sqrt(kEps), p->policy, nullptr, nullptr, 0,
-                                     nullptr, p->cap);
+                                     nullptr, p->gram_rows, p->cap);
       HIPCHK(hipStreamEndCapture(p->cap, &p->graph));
       if (lr != 0) {
         batch_release_graph(p);

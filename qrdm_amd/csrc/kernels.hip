@@ -35,8 +35,8 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
 // Two-norm of x[0..n) by one wave; every lane returns it.  Plain sum of
 // squares, redone with scaling by max|x| when the sum is out of the range
 // where squaring is exact enough (the role of dnrm2's scaling).
-__device__ double wave_nrm2(const double* __restrict__ x, int n) {
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ double wave_nrm2(const double* __restrict__ x, int n) {
+  const int lane = local_tid() & 63;
   // four independent loads and partial sums per lane per iteration, so the
   // loads of one column are in flight together
   double s0 = 0.0, s1 = 0.0, s2a = 0.0, s3 = 0.0;
@@ -69,7 +69,7 @@ __device__ double wave_nrm2(const double* __restrict__ x, int n) {
 
 // Block-wide sum; red must hold blockDim.x/64 doubles.  All threads return it.
 __device__ double block_sum(double v, double* red) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int lane = local_tid() & 63, wave = local_tid() >> 6, nw = blockDim.x >> 6;
   v = wave_sum(v);
   __syncthreads();
   if (lane == 0) red[wave] = v;
@@ -79,8 +79,8 @@ __device__ double block_sum(double v, double* red) {
   return t;
 }
 
-__device__ double block_max(double v, double* red) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = local_tid() & 63, wave = local_tid() >> 6, nw = blockDim.x >> 6;
   v = wave_max(v);
   __syncthreads();
   if (lane == 0) red[wave] = v;
@@ -94,7 +94,7 @@ __device__ double block_max(double v, double* red) {
 __device__ double block_nrm2(const double* __restrict__ x, int n, double* red) {
   const int nt = blockDim.x;
   double s0 = 0.0, s1 = 0.0;
-  int i = threadIdx.x;
+  int i = local_tid();
   for (; i + nt < n; i += 2 * nt) {
     const double v0 = x[i], v1 = x[i + nt];
     s0 = fma(v0, v0, s0);
@@ -108,11 +108,11 @@ __device__ double block_nrm2(const double* __restrict__ x, int n, double* red) {
   if (isnan(s)) return s;
   if (s >= 1e-280 && s <= 1e280) return sqrt(s);
   double amax = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) amax = fmax(amax, fabs(x[i]));
+  for (int i = local_tid(); i < n; i += blockDim.x) amax = fmax(amax, fabs(x[i]));
   amax = block_max(amax, red);
   if (amax == 0.0 || isinf(amax)) return amax;
   double r = 1.0 / amax, s2 = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = local_tid(); i < n; i += blockDim.x) {
     double v = x[i] * r;
     s2 = fma(v, v, s2);
   }
@@ -145,13 +145,13 @@ constexpr int NORM_COLS = 4;
 __device__ __forceinline__ void block_max_atomic(double v, int bad, DevState* st) {
   __shared__ double wmax[4];
   __shared__ int wbad[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wave = local_tid() >> 6;
   if (lane == 0) {
     wmax[wave] = v;
     wbad[wave] = bad;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (local_tid() == 0) {
     const double mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
     const int b = wbad[0] | wbad[1] | wbad[2] | wbad[3];
     if (b) atomicOr(&st->flags, 1);
@@ -199,7 +199,7 @@ __device__ __forceinline__ bool key_before(double va, int ia, double vb, int ib)
 // permutations for JJ < 16, the gfx950 row-swap instructions above.
 template <int JJ>
 __device__ __forceinline__ int xor_lane(int x) {
-  const int lane = threadIdx.x & 63;
+  const int lane = local_tid() & 63;
   if constexpr (JJ == 1) {
     return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
   } else if constexpr (JJ == 2) {
@@ -245,7 +245,7 @@ __device__ __forceinline__ void lane_cx(double& v, int& ix, bool want_first) {
 // network, blocks of K keys merged with distances K/2 .. 1.
 template <int K, int JJ>
 __device__ __forceinline__ void sort_stages(double& v, int& ix) {
-  const int lane = threadIdx.x & 63;
+  const int lane = local_tid() & 63;
   const bool first = (lane & JJ) == 0;       // lower position of the pair
   const bool desc = (lane & (K & 63)) == 0;  // this K-block sorts descending
   lane_cx<JJ>(v, ix, first == desc);
@@ -262,7 +262,7 @@ __device__ __forceinline__ void wave_sort64(double& v, int& ix) {
 // (bv, bi), sorted: the lane-wise better of a[lane] and b[63 - lane] is a
 // bitonic sequence holding them, which six half-cleaner stages sort.
 __device__ __forceinline__ void wave_merge64(double& v, int& ix, double bv, int bi) {
-  const int lane = threadIdx.x & 63;
+  const int lane = local_tid() & 63;
   // b[63 - lane]: lane ^ 63 through the six distances
   bv = xor_lane<32>(bv);
   bi = xor_lane<32>(bi);
@@ -293,7 +293,7 @@ __device__ __forceinline__ void wave_merge64(double& v, int& ix, double bv, int 
 template <int NW>
 __device__ __forceinline__ void block_merge_lists(double (*lv)[KMAX], int (*li)[KMAX], double& v,
                                                   int& ix) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wave = local_tid() >> 6;
 #pragma unroll
   for (int h = NW / 2; h >= 1; h >>= 1) {
     // wave w < h merges lists w and w + h into list w
@@ -482,7 +482,7 @@ void launch_topk(const double* work, int j0, int m, int n, int nb, double tau, d
 // X[c * (GRAM_ROWS + 1) + r]; 256 threads, wave w computes rows 16w..16w+15
 // on the matrix cores.
 __device__ __forceinline__ void gram_slab_acc(const double* X, dbl4 (&acc)[4]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wv = local_tid() >> 6;
   const int g = lane >> 4, li = lane & 15;
 #pragma unroll 4
   for (int k0 = 0; k0 < GRAM_ROWS; k0 += 4) {
@@ -495,7 +495,7 @@ __device__ __forceinline__ void gram_slab_acc(const double* X, dbl4 (&acc)[4]) {
 }
 
 __device__ __forceinline__ void gram_store(const dbl4 (&acc)[4], double* out) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wv = local_tid() >> 6;
   const int g = lane >> 4, li = lane & 15;
 #pragma unroll
   for (int u = 0; u < 4; u++)
@@ -607,12 +607,12 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // Run by one wave (threadIdx.x < 64); its LDS steps are ordered by
 // wave_lds_sync, so the rest of the workgroup may be elsewhere.  Gs: the
 // candidate Gram in LDS, column-major with leading dimension KMAX + 1.
-__device__ void select_wave(const double* Gs, double delta, int m, int n, int fast_rows,
+__device__ __forceinline__ void select_wave(const double* Gs, double delta, int m, int n, int fast_rows,
                             int exact_queued, int* perm, DevState* st) {
   __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
   __shared__ int s_occ[KMAX], s_slot[KMAX];
-  const int t = threadIdx.x;
+  const int t = local_tid();
   const int j = st->j, nc = st->nc, kmax = st->kmax;
   const int nsub = n - j;
   if (st->policy_on) delta = st->delta_step;
@@ -906,7 +906,7 @@ __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp
     // an entry before its writes; npair <= 2 KMAX <= the block's threads)
     double* Wq = w_prev(bufs, st);
     for (int i = blk; i < KMAX; i += nblk) {
-      const int p = threadIdx.x;
+      const int p = local_tid();
       const double v = (p < npair) ? Wq[(size_t)perm[2 * p + 1] * KMAX + i] : 0.0;
       __syncthreads();
       if (p < npair) Wq[(size_t)perm[2 * p] * KMAX + i] = v;
@@ -921,31 +921,31 @@ __device__ void apply_perm_rows(double* A, int lda, int m, double* work, int* jp
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int t = threadIdx.x + 256 * (q0 + q);
+      const int t = local_tid() + 256 * (q0 + q);
       const int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
       v[q] = (t < tot && r < m) ? A[(size_t)perm[2 * p + 1] * lda + r] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int t = threadIdx.x + 256 * (q0 + q);
+      const int t = local_tid() + 256 * (q0 + q);
       if (t < tot) buf[t] = v[q];
     }
   }
   double wv = 0.0;
   int jv = 0;
-  const bool side = blk == 0 && threadIdx.x < npair;
+  const bool side = blk == 0 && local_tid() < npair;
   if (side) {
-    wv = work[perm[2 * threadIdx.x + 1]];
-    if (jp) jv = jp[perm[2 * threadIdx.x + 1]];
+    wv = work[perm[2 * local_tid() + 1]];
+    if (jp) jv = jp[perm[2 * local_tid() + 1]];
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < tot; t += 256) {
+  for (int t = local_tid(); t < tot; t += 256) {
     const int p = t / PERM_ROWS, r = rs + t % PERM_ROWS;
     if (r < m) A[(size_t)perm[2 * p] * lda + r] = buf[t];
   }
   if (side) {
-    work[perm[2 * threadIdx.x]] = wv;
-    if (jp) jp[perm[2 * threadIdx.x]] = jv;
+    work[perm[2 * local_tid()]] = wv;
+    if (jp) jp[perm[2 * local_tid()]] = jv;
   }
 }
 
@@ -975,10 +975,10 @@ void launch_apply_perm(double* A, int lda, int m, double* work, int* jp, const i
 // GROUP: block columns a wave updates at a time (1 for many-wave workgroups,
 // 4 for few waves).
 template <int GROUP>
-__device__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, double tau_dm,
+__device__ __forceinline__ int panel_exact_body(double* A, int lda, int j, int m, int fjb, double tau_dm,
                                 int truncate, double* tau, double* red) {
   const int msub = m - j;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
   double* P = A + j + (size_t)j * lda;
   double thres = 5e-14;
   int i;
@@ -1184,9 +1184,9 @@ void launch_vgram_exact(const double* A, int lda, int j0, int m, const DevState*
 // The recurrence for the whole workgroup (at least 64 threads): Ss = V^T V
 // and Ts (zero on entry) column-major with leading dimension KMAX, ts the
 // kk scalars tau, tmp KMAX doubles; all in LDS.
-__device__ void larft_recurrence(const double* Ss, double* Ts, const double* ts, double* tmp,
+__device__ __forceinline__ void larft_recurrence(const double* Ss, double* Ts, const double* ts, double* tmp,
                                  int kk) {
-  const int p = threadIdx.x;
+  const int p = local_tid();
   for (int i = 0; i < kk; i++) {
     const double ti = ts[i];
     if (ti != 0.0) {

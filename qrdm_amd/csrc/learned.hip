@@ -53,9 +53,9 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 // thread calls it; it ends with a barrier); work holds the n - j partial
 // norms of the remaining columns, maxw their maximum; hist is LDS scratch of
 // POL_BINS ints.
-__device__ void policy_features(const double* work, int nsub, double maxw, int j, int m, int n,
+__device__ __forceinline__ void policy_features(const double* work, int nsub, double maxw, int j, int m, int n,
                                 float* feat, int* hist) {
-  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tid = local_tid(), nt = blockDim.x;
   if (tid < POL_BINS) hist[tid] = 0;
   __syncthreads();
   for (int t = tid; t < nsub; t += nt) {
@@ -76,9 +76,9 @@ __device__ void policy_features(const double* work, int nsub, double maxw, int j
 
 // Forward pass by one wave (lane = hidden unit); feat and hbuf (64 floats)
 // in LDS.  Returns the two sigmoid outputs in every lane.
-__device__ float2 policy_forward(const PolicyNet* __restrict__ P, const float* feat,
+__device__ __forceinline__ float2 policy_forward(const PolicyNet* __restrict__ P, const float* feat,
                                  float* hbuf) {
-  const int l = threadIdx.x & 63;
+  const int l = local_tid() & 63;
   float z1 = P->b1[l];
 #pragma unroll 2
   for (int k = 0; k < POL_IN; k++) z1 = fmaf(bf16_bits_to_float(P->w1[l * POL_IN + k]), feat[k], z1);
@@ -309,7 +309,7 @@ __device__ constexpr double kLblDelta[LBL_ND] = {0.80, 0.90, 0.95, 0.98};
 
 __device__ int label_block(const double* G, const double* cand, int kmax, double tau,
                            double delta, double tau_def, double* L) {
-  const int t = threadIdx.x & 63;
+  const int t = local_tid() & 63;
   const double maxw = cand[0];
   const double vt = cand[t];
   const int nc = __popcll(__ballot(t < kmax && vt > tau * maxw));
@@ -365,11 +365,11 @@ __device__ int label_block(const double* G, const double* cand, int kmax, double
 // normalised to [0, 1] over the output ranges, the default's block, the
 // label's block, j, n - j) to rec.  feat must hold the inputs.
 constexpr int POL_REC = 40;
-__device__ void label_record(const double* G, const double* cand, int kmax, double tau_def,
+__device__ __forceinline__ void label_record(const double* G, const double* cand, int kmax, double tau_def,
                              double delta_def, double* L, const float* feat, int j, int nsub,
                              float tau_lo, float tau_hi, float delta_lo, float delta_hi,
                              float* rec) {
-  const int t = threadIdx.x & 63;
+  const int t = local_tid() & 63;
   const int eff_def = label_block(G, cand, kmax, tau_def, delta_def, tau_def, L);
   int best = eff_def;
   double bt = tau_def, bd = delta_def;

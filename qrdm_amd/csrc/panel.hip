@@ -91,7 +91,7 @@ __device__ __forceinline__ double rcp_nr12(double x) {
 // bs..bs+15 and columns bs..bs+15 read.
 __device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, int nlim,
                                               bool sym) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wave = local_tid() >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int nt = (KMAX - lo) >> 4;
   int e = 0;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, 
 template <class StopFn>
 __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn stopfn, double* rinv,
                                             int* s_done, int* s_why, int* why) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
   for (int bs = 0; bs < nlim; bs += BLK) {
     if (wave == 0) {
       // lane r (= lane % 16) holds row bs + r of the symmetric diagonal block;
@@ -211,7 +211,7 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
                                                  double thres, double* dsg, double* tl,
                                                  double* scv, int* s_done, int* s_why,
                                                  int* why) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
   for (int bs = 0; bs < nlim; bs += BLK) {
     if (wave == 0) {
       // lane r (= lane % 16) holds row bs + r of the diagonal block; the pivot
@@ -333,7 +333,7 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
 template <bool BOTH = true>
 __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[LDP], double* rinv,
                                            double (*Ys)[BLK][BLK + 1]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
   // R_m(i, l): m = 1 U, m = 0 L^T (unit diagonal); zero below the diagonal
   auto Rm = [&](int m, int i, int l) -> double {
     return m ? ((i <= l) ? W[i][l] : 0.0) : ((i < l) ? W[l][i] : (i == l ? 1.0 : 0.0));
@@ -405,9 +405,51 @@ __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[L
 }
 
 // ---------------------------------------------------------------------------
-// R1 = chol(G) of the panel's Gram (gathered from the candidate Gram through
-// the rank map), stopped at the first pivot that proves the DM truncation
-// fires there; R1^-1.  Identity padding outside fjb.
+// R1 = chol(G) of the panel's Gram, held in S (identity beyond fjb; diag[]:
+// its diagonal), stopped at the first pivot that proves the DM truncation
+// fires there, and R1^-1.  Returns k1 >= 0 with S = R1 (on and above the
+// diagonal, zero below) identity-padded beyond k1 and X = R1^-1 (upper,
+// identity beyond k1); or -1 when a pivot does not resolve (the exact panel
+// takes the step over).  Y: lu_inverse's scratch.  The caller's S and diag
+// writes are ordered by a barrier before the call.
+__device__ __forceinline__ int chol1_core(double (*S)[LDP], double (*X)[LDP], double (*Y)[BLK][BLK + 1],
+                          double* rinv, const double* diag, int fjb, double tau_dm, int* s_done,
+                          int* s_why) {
+  const int tid = local_tid();
+  double maxd = 0.0;
+  for (int q = 0; q < KMAX; q++) maxd = fmax(maxd, diag[q]);
+  const double g00 = diag[0];
+  const double tolp = 1e4 * 64.0 * kEpsP * fjb * maxd;  // unresolved pivots
+  const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * sqrt(g00) : 5e-14;
+  const double thr2 = 0.25 * thres * thres;
+  const bool scale_ok = g00 > 1e-280 && g00 < 1e280 && maxd < 1e280;
+  lds_barrier();
+  int why = 0;
+  const int k1 = chol_blocked(
+      S, fjb,
+      [&](int i, double d) -> int {
+        if (!scale_ok) return 2;
+        if (i >= 1 && d < thr2 && thr2 > tolp) return 1;  // truncation at <= i is certain
+        if (!(d > tolp)) return 2;
+        return 0;
+      },
+      rinv, s_done, s_why, &why);
+  if (why == 2) return -1;
+  // R1 (zero below the diagonal), identity-padded beyond k1 for the inverse
+  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    const int r = e >> 6, c = e & 63;
+    const bool in = r < k1 && c < k1;
+    const double v = (in && c >= r) ? S[r][c] : 0.0;
+    S[r][c] = in ? v : (r == c ? 1.0 : 0.0);
+  }
+  lds_barrier();
+  lu_inverse<false>(S, X, rinv, Y);  // X = R1^-1 (upper part)
+  return k1;
+}
+
+// chol1_core on the panel's Gram gathered from the candidate Gram Gc
+// (column-major, leading dimension KMAX) through the rank map; R1 and R1^-1
+// out (column-major, zero outside k1 x k1), or the fallback flag.
 __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict__ rank,
                            double tau_dm, double* R1, double* R1inv, DevState* st) {
   __shared__ double S[KMAX][LDP];
@@ -438,44 +480,19 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
       S[e >> 6][e & 63] = gv[q];
     }
   }
-  double maxd = 0.0;
-  for (int q = 0; q < KMAX; q++) maxd = fmax(maxd, diag[q]);
-  const double g00 = diag[0];
-  const double tolp = 1e4 * 64.0 * kEpsP * fjb * maxd;  // unresolved pivots
-  const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * sqrt(g00) : 5e-14;
-  const double thr2 = 0.25 * thres * thres;
-  const bool scale_ok = g00 > 1e-280 && g00 < 1e280 && maxd < 1e280;
-  lds_barrier();
-  int why = 0;
-  const int k1 = chol_blocked(
-      S, fjb,
-      [&](int i, double d) -> int {
-        if (!scale_ok) return 2;
-        if (i >= 1 && d < thr2 && thr2 > tolp) return 1;  // truncation at <= i is certain
-        if (!(d > tolp)) return 2;
-        return 0;
-      },
-      rinv, &s_done, &s_why, &why);
-  if (why == 2) {
+  const int k1 = chol1_core(S, X, Y, rinv, diag, fjb, tau_dm, &s_done, &s_why);
+  if (k1 < 0) {
     if (tid == 0) {
       st->flags |= 2;  // the exact panel takes the step over
       st->halt = 2;
     }
     return;
   }
-  // R1 (zero outside k1) out; identity-padded R1 in S for the inverse
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
     const int r = e >> 6, c = e & 63;
     const bool in = r < k1 && c < k1;
-    const double v = (in && c >= r) ? S[r][c] : 0.0;
-    R1[r + c * KMAX] = v;
-    S[r][c] = in ? v : (r == c ? 1.0 : 0.0);
-  }
-  lds_barrier();
-  lu_inverse<false>(S, X, rinv, Y);  // X = R1^-1 (upper part)
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
-    const int r = e >> 6, c = e & 63;
-    R1inv[r + c * KMAX] = (r < k1 && c < k1) ? X[r][c] : 0.0;
+    R1[r + c * KMAX] = (in && c >= r) ? S[r][c] : 0.0;
+    R1inv[r + c * KMAX] = in ? X[r][c] : 0.0;
   }
   if (tid == 0) st->k1 = k1;
 }
@@ -507,7 +524,7 @@ void launch_perm_chol1(double* A, int lda, int m, double* work, int* jp, const i
 // acc[u][v] holds row 16w + lane/16 + 4v, column 16u + lane%16.
 __device__ __forceinline__ void mfma_64(dbl4p (&acc)[4], const double (*X)[LDP], bool xt,
                                         const double (*Y)[LDP]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wv = local_tid() >> 6;
 #pragma unroll
   for (int u = 0; u < 4; u++) acc[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
   const int ri = 16 * wv + (lane & 15), kq = lane >> 4;
@@ -523,7 +540,7 @@ __device__ __forceinline__ void mfma_64(dbl4p (&acc)[4], const double (*X)[LDP],
 }
 
 __device__ __forceinline__ void mfma_store(double (*S)[LDP], const dbl4p (&acc)[4]) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = local_tid() & 63, wv = local_tid() >> 6;
 #pragma unroll
   for (int u = 0; u < 4; u++)
 #pragma unroll
@@ -573,7 +590,7 @@ __device__ __forceinline__ int tri_jobs(int kind, int wave, TileJob (&jb)[4]) {
 template <class FA, class FB>
 __device__ __forceinline__ int tri_product(int kind, FA fa, FB fb, TileJob (&jb)[4],
                                            dbl4p (&acc)[4]) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const int lane = local_tid() & 63, wave = local_tid() >> 6, g = lane >> 4, li = lane & 15;
   const int nj = tri_jobs(kind, wave, jb);
 #pragma unroll
   for (int t = 0; t < 4; t++) {
@@ -597,7 +614,7 @@ __device__ __forceinline__ int tri_product(int kind, FA fa, FB fb, TileJob (&jb)
 // slab of the panel); 256 threads, all loads in flight before the LDS writes.
 __device__ __forceinline__ void load_slab(double (*Ps)[LDP], const double* __restrict__ A,
                                           int lda, int j, int rs, int m, int ncol) {
-  const int tid = threadIdx.x;
+  const int tid = local_tid();
   double v[KMAX * PR_ROWS / 256];
 #pragma unroll
   for (int q = 0; q < KMAX * PR_ROWS / 256; q++) {
@@ -613,7 +630,7 @@ __device__ __forceinline__ void load_slab(double (*Ps)[LDP], const double* __res
 
 // S[r][c] = M[r + c * KMAX] for a column-major 64 x 64 matrix in memory.
 __device__ __forceinline__ void load_mat(double (*S)[LDP], const double* __restrict__ M) {
-  const int tid = threadIdx.x;
+  const int tid = local_tid();
   double v[KMAX * KMAX / 256];
 #pragma unroll
   for (int q = 0; q < KMAX * KMAX / 256; q++) v[q] = M[tid + 256 * q];
@@ -655,42 +672,20 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
 }
 
 // ---------------------------------------------------------------------------
-// Reconstruction: see the file header.  Writes the top k x k block of the
-// panel (R above and on the diagonal, V1 below), tau, T and M; or only the
-// fallback flag.  st->fjb_cmp = k.
-__device__ void recon_body(const double* __restrict__ G2, const double* __restrict__ R1,
-                           const double* __restrict__ R1inv, double* A, int lda, double tau_dm,
-                           double* tau, double* T_out, double* M_out, DevState* st) {
-  __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
-  __shared__ double SA[KMAX][LDP];  // R^-1
-  __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
-  __shared__ double SX[KMAX][LDP];  // G2, F or R2, R1^-1, then U
-  __shared__ double Y[4][BLK][BLK + 1];
-  __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
-  __shared__ int s_done, s_why;
-  if (!step_runs(st, RUN_FAST)) return;
-  const int j = st->j, fjb = st->fjb;
-  tau += j;
-  const int tid = threadIdx.x;
-  const int k1 = st->k1;
-  {
-    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256];
-#pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      // e walks rows fastest: coalesced column-major reads
-      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
-      const bool in = r < k1 && c < k1;
-      gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);  // from the launch's reduce
-      gr[q] = R1[r + c * KMAX];
-    }
-#pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
-      SX[r][c] = gx[q];
-      SR[r][c] = gr[q];
-    }
-  }
-  lds_barrier();
+// Reconstruction (see the file header) on LDS matrices: SX = G2 (identity
+// beyond k1), SR = R1 (zero beyond k1); R1inv: R1^-1 in memory (column-major,
+// leading dimension KMAX, zero outside k1 x k1), or nullptr when SA already
+// holds it.  Writes the top k x k block of the panel at column j (R above and
+// on the diagonal, V1 below), tau[0..k) and M (column-major, leading
+// dimension KMAX) to M_out, and returns k; or returns -1, with nothing
+// written, when the Gram-based form falls back.  Y may share SW's storage.
+// The caller's LDS writes are ordered by a barrier before the call.
+__device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], double (*SW)[LDP],
+                          double (*SX)[LDP], double (*Y)[BLK][BLK + 1], double* rinv,
+                          double* dsg, double* tl, double* scv, int* s_done, int* s_why,
+                          const double* R1inv, double* A, int lda, int j, int fjb, int k1,
+                          double tau_dm, double* tau, double* M_out) {
+  const int tid = local_tid();
   // 1-3. R = R2 R1 with R2^T R2 = G2 and R^-1 (into SR and SA).  After the
   // first pass G2 = I + E with E tiny: then R2 = I + F, F upper triangular
   // with F + F^T + F^T F = E, is the fixed point of F = triu_half(E - F^T F)
@@ -738,9 +733,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
           TRI_LU, [&](int r, int q) { return SX[q][r]; }, [&](int q, int c) { return SX[q][c]; },
           jb, acc);
       lds_barrier();
-      for (int t = 0; t < nj; t++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
+      for (int t = 0; t < 4; t++)  // static indices into jb and acc
+#pragma unroll
+        for (int v = 0; v < 4; v++) if (t < nj) {
           const int r = BLK * jb[t].ti + g + 4 * v, c = BLK * jb[t].tj + li;
           const double x = SW[r][c] - acc[t][v];
           SX[r][c] = (r < c) ? x : (r == c ? 0.5 * x : 0.0);
@@ -753,9 +749,10 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
           TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SR[q][c]; },
           jb, acc);
       lds_barrier();
-      for (int t = 0; t < nj; t++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) SR[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] += acc[t][v];
+      for (int t = 0; t < 4; t++)  // static indices into jb and acc
+#pragma unroll
+        for (int v = 0; v < 4; v++) if (t < nj) SR[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] += acc[t][v];
     }
     // R2^-1 = I - F (I - F (...)) into SW (upper)
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
@@ -768,16 +765,22 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
           TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SW[q][c]; },
           jb, acc);
       lds_barrier();
-      for (int t = 0; t < nj; t++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
+      for (int t = 0; t < 4; t++)  // static indices into jb and acc
+#pragma unroll
+        for (int v = 0; v < 4; v++) if (t < nj) {
           const int r = BLK * jb[t].ti + g + 4 * v, c = BLK * jb[t].tj + li;
           SW[r][c] = (r == c ? 1.0 : 0.0) - acc[t][v];
         }
       lds_barrier();
     }
     // R^-1 = R1^-1 R2^-1 (both upper) into SA, zero below the diagonal blocks
-    load_mat(SX, R1inv);
+    if (R1inv) {
+      load_mat(SX, R1inv);
+    } else {  // SA holds R1^-1: SX takes it before SA is overwritten
+      for (int e = tid; e < KMAX * KMAX; e += blockDim.x) SX[e & 63][e >> 6] = SA[e & 63][e >> 6];
+      lds_barrier();
+    }
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
       const int r = e & 63, c = e >> 6;
       if ((r >> 4) > (c >> 4)) SA[r][c] = 0.0;
@@ -787,22 +790,17 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       const int nj = tri_product(
           TRI_UU, [&](int r, int q) { return SX[r][q]; }, [&](int q, int c) { return SW[q][c]; },
           jb, acc);
-      for (int t = 0; t < nj; t++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) SA[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] = acc[t][v];
+      for (int t = 0; t < 4; t++)  // static indices into jb and acc
+#pragma unroll
+        for (int v = 0; v < 4; v++) if (t < nj) SA[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] = acc[t][v];
     }
     lds_barrier();
   } else {
     chol_blocked(
         SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv,
-        &s_done, &s_why, &why);
-    if (why) {
-      if (tid == 0) {
-        st->flags |= 2;  // the exact panel takes the step over
-        st->halt = 2;
-      }
-      return;
-    }
+        s_done, s_why, &why);
+    if (why) return -1;
     for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
       const int r = e >> 6, c = e & 63;
       if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
@@ -834,14 +832,8 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   lds_barrier();
   // 5. LU of Q11 D - I with dlarfg's signs and the truncation test
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * SR[0][0] : 5e-14;
-  const int k = lu_signed_blocked(SW, k1, SR, thres, dsg, tl, scv, &s_done, &s_why, &why);
-  if (why == 2) {
-    if (tid == 0) {
-      st->flags |= 2;  // the exact panel takes the step over
-      st->halt = 2;
-    }
-    return;
-  }
+  const int k = lu_signed_blocked(SW, k1, SR, thres, dsg, tl, scv, s_done, s_why, &why);
+  if (why == 2) return -1;
   // 6. top block (R rows scaled by the signs, V1 below), tau; SX := the
   // signed U (on and above the diagonal) and L (below), identity beyond k
   for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
@@ -892,6 +884,52 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
       if ((r >> 4) > (c >> 4)) M_out[r + c * KMAX] = 0.0;
     }
   }
+  return k;
+}
+
+// The reconstruction of the DM loop's Gram-based panel: G2, R1 and R1^-1 from
+// memory, st->fjb_cmp and st->c0, or the fallback flag.
+__device__ void recon_body(const double* __restrict__ G2, const double* __restrict__ R1,
+                           const double* __restrict__ R1inv, double* A, int lda, double tau_dm,
+                           double* tau, DevState* st, double* M_out) {
+  __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
+  __shared__ double SA[KMAX][LDP];  // R^-1
+  __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
+  __shared__ double SX[KMAX][LDP];  // G2, F or R2, R1^-1, then U
+  __shared__ double Y[4][BLK][BLK + 1];
+  __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
+  __shared__ int s_done, s_why;
+  if (!step_runs(st, RUN_FAST)) return;
+  const int j = st->j, fjb = st->fjb;
+  const int tid = threadIdx.x;
+  const int k1 = st->k1;
+  {
+    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      // e walks rows fastest: coalesced column-major reads
+      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+      const bool in = r < k1 && c < k1;
+      gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);  // from the launch's reduce
+      gr[q] = R1[r + c * KMAX];
+    }
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / 256; q++) {
+      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+      SX[r][c] = gx[q];
+      SR[r][c] = gr[q];
+    }
+  }
+  lds_barrier();
+  const int k = recon_core(SR, SA, SW, SX, Y, rinv, dsg, tl, scv, &s_done, &s_why, R1inv, A, lda,
+                           j, fjb, k1, tau_dm, tau + j, M_out);
+  if (k < 0) {
+    if (tid == 0) {
+      st->flags |= 2;  // the exact panel takes the step over
+      st->halt = 2;
+    }
+    return;
+  }
   if (tid == 0) {
     st->fjb_cmp = k;
     // the trailing update also applies the kept reflectors to the rejected
@@ -937,7 +975,7 @@ __global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__
   if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  recon_body(G2, R1, R1inv, A, lda, tau_dm, tau, T_out, M_out, st);
+  recon_body(G2, R1, R1inv, A, lda, tau_dm, tau, st, M_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -97,6 +97,18 @@ __device__ __forceinline__ bool update_pending(const DevState* st) {
   return st->fused && st->it > 0;
 }
 
+// The workitem id through an empty asm: what a helper derives from it (LDS
+// offsets, lane masks) is computed where the helper runs, not hoisted to the
+// top of a persistent kernel's loop and held in registers across its other
+// phases.
+__device__ __forceinline__ int local_tid() {
+  int t = threadIdx.x;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(t));
+#endif
+  return t;
+}
+
 // Value of x in lane l of the wave (l a compile-time or wave-uniform index).
 __device__ __forceinline__ double rdlane(double x, int l) {
   const long long b = __double_as_longlong(x);
@@ -247,12 +259,15 @@ void launch_apply_prec(int prec, double* A, int lda, int j, int m, int c0, int n
 // nullptr; trace: per-step training records, or nullptr (ranges: the label
 // normalisation tau_lo, tau_hi, delta_lo, delta_hi).
 constexpr int BATCH_NMAX = 1024;
+// gram_rows: panels of at least that many rows (and two or more columns) take
+// the Gram-based form (panel.hip's, inside the workgroup), the others the
+// column-by-column one; <= 0: column by column only.
 // Returns -1 (nothing launched) for a launch shape other than 4 waves.
 int launch_dm_batch(int waves, int grid, const void* items, const int* order, int count,
                      int* queue, double* A, double* tau, int* jpvt, int* ncols,
                      int* info, int nb, double delta, double tau_dm, int stop_mode, double eta3,
                      double tol3z, const void* policy, float* trace, unsigned* trace_count,
-                     int trace_cap, const float* ranges, hipStream_t s);
+                     int trace_cap, const float* ranges, int gram_rows, hipStream_t s);
 // The learned (tau, delta) proposal (learned.hip).
 int policy_nparam();
 size_t policy_net_bytes();

@@ -81,6 +81,7 @@ int main(int argc, char** argv) {
   check(qrdm_batch_plan_set_shape(p, 8) == -1, "8 waves refused", 0);
   check(qrdm_batch_plan_set_shape(p, 4) == 0, "4 waves", 0);
   const double th[3] = {0.9, 0.15, 0.0};
+  check(qrdm_batch_plan_set_panel_rows(p, 128) == 0, "panel rows", 0);
   check(qrdm_batch_autotune(p, src, th, nullptr) == 0, "autotune", 0);
   for (int rep = 0; rep < 4; rep++) {
     memcpy(work, src, sizeof(double) * a);
@@ -95,8 +96,8 @@ int main(int argc, char** argv) {
     while (k < items[i].n && s < mn) s += ncb[items[i].col_off + k++];
     check(s == mn && info[i] == 0, "batch item", i);
   }
-  int q[4];
-  check(qrdm_batch_plan_query(p, q, 4) == 4 && q[0] == 4, "query", q[0]);
+  int q[5];
+  check(qrdm_batch_plan_query(p, q, 5) == 5 && q[0] == 4, "query", q[0]);
 
   // ---- the per-matrix loop over the same matrices
   for (int i = 0; i < cnt; i++) {

@@ -405,7 +405,7 @@ void launch_wide_state(DevState* st, int j, int kk, hipStream_t s) {
 int launch_dm_batch(int waves, int, const void* items, const int* order, int count, int* queue,
                     double*, double* tau, int* jpvt, int* ncols, int* info, int nb, double,
                     double, int, double, double, const void*, float*, unsigned*, int,
-                    const float*, hipStream_t s) {
+                    const float*, int, hipStream_t s) {
   if (waves != 4) return -1;
   submit(s, [=] {
     const qrdm_batch_item* it = static_cast<const qrdm_batch_item*>(items);

@@ -108,17 +108,19 @@ def check(got, ref, lead):
     assert rel(np.triu(got["a"][:lead, :lead]), np.triu(ref["a"][:lead, :lead])) <= 1e-12
 
 
-@pytest.mark.parametrize("waves", [4, pytest.param(8, marks=pytest.mark.skip(
+@pytest.mark.parametrize("waves,rows", [(4, 0), (4, 128), (4, 64), pytest.param(8, 0, marks=pytest.mark.skip(
     reason="8-wave shape switched off: runs of it on a 4096-matrix batch ended in "
            "host-heap aborts of unidentified cause"))])
-def test_batched_vs_oracle(gpu, oracle, waves):
+def test_batched_vs_oracle(gpu, oracle, waves, rows):
+    """rows > 0: panels of at least that many rows take the Gram-based form."""
     import qrdm_amd
     torch = gpu
     mats, leads = _mixed_small(7, 36)
     plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=64)
     plan.set_shape(waves)
-    assert plan.query()["waves"] == waves
-    say("batch of %d, waves %d" % (len(mats), waves))
+    plan.set_panel_rows(rows)
+    assert plan.query()["waves"] == waves and plan.query()["panel_rows"] == rows
+    say("batch of %d, waves %d, panel rows %d" % (len(mats), waves, rows))
     buf, tau, jp, nc, info = run_batch(gpu, plan, pack(torch, plan, mats))
     for i, a in enumerate(mats):
         got = unpack(plan, i, buf, tau, jp, nc, info)
@@ -130,7 +132,10 @@ def test_batched_vs_oracle(gpu, oracle, waves):
             raise
 
 
-def test_batched_nb_and_stop_modes(gpu, oracle):
+@pytest.mark.parametrize("rows", [0, 64])
+def test_batched_nb_and_stop_modes(gpu, oracle, rows):
+    """Rank-20 matrices with noise: the panels truncate (inside the
+    Gram-based form too, rows = 64), and the stop modes end early."""
     import qrdm_amd
     torch = gpu
     rng = np.random.default_rng(3)
@@ -139,6 +144,7 @@ def test_batched_nb_and_stop_modes(gpu, oracle):
     for nb in (16, 64):
         for stop, eta in ((0, 0.0), (1, 0.0), (3, 1e-8)):
             plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=nb)
+            plan.set_panel_rows(rows)
             buf, tau, jp, nc, info = run_batch(gpu, plan, pack(torch, plan, mats),
                                                thres=(0.9, 0.15, eta), stop=stop)
             for i, a in enumerate(mats):
@@ -155,11 +161,13 @@ def test_batched_nb_and_stop_modes(gpu, oracle):
             plan.close()
 
 
-def test_batched_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("rows", [0, 128])
+def test_batched_graph_replays_match_eager(gpu, rows):
     import qrdm_amd
     torch = gpu
     mats, _ = _mixed_small(11, 20)
     plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=64)
+    plan.set_panel_rows(rows)
     src = pack(torch, plan, mats)
     work = src.clone()
     eager = run_batch(gpu, plan, work, graph=False)
@@ -198,9 +206,9 @@ def test_batched_nonfinite_and_bad_items(gpu):
 
 
 def test_batched_autotune_keeps_results(gpu, oracle):
-    """Autotuning (nb among 16, 32, 64 for a plan made with nb <= 0) leaves
-    the input untouched and the results equal to the oracle's at the nb it
-    picked."""
+    """Autotuning (nb among 16, 32, 64 for a plan made with nb <= 0, panel
+    rows among 0, 128, 256) leaves the input untouched and the results equal
+    to the oracle's at the nb it picked."""
     import qrdm_amd
     torch = gpu
     mats, leads = _mixed_small(21, 24)
@@ -209,11 +217,48 @@ def test_batched_autotune_keeps_results(gpu, oracle):
     before = src.cpu().numpy()
     q = plan.autotune(src)
     say("autotune:", q)
-    assert q["nb"] in (16, 32, 64) and q["waves"] == 4 and q["tuned"] == 3
+    assert q["nb"] in (16, 32, 64) and q["waves"] == 4 and q["tuned"] == 9
+    assert q["panel_rows"] in (0, 128, 256)
     assert np.array_equal(src.cpu().numpy(), before)
     buf, tau, jp, nc, info = run_batch(gpu, plan, src, graph=True)
     for i, a in enumerate(mats):
         check(unpack(plan, i, buf, tau, jp, nc, info), oracle.dgeqrdm(a, nb=q["nb"]), leads[i])
+
+
+def test_batched_gram_panel_tall_uneven(gpu, oracle):
+    """Tall uneven matrices (m up to 1024, the batched workload's range) with
+    the Gram-based panel on every panel of 128 rows or more, against the
+    oracle: Gaussian and graded inputs at the full bar, low-rank ones
+    (truncation inside the Gram-based form) on their resolved pivots."""
+    import qrdm_amd
+    torch = gpu
+    rng = np.random.default_rng(17)
+    mats, leads = [], []
+    for i in range(24):
+        m = int(rng.integers(300, 1025))
+        n = int(rng.integers(64, min(m, 400) + 1))
+        if i % 3 == 0:
+            mats.append(inputs.gaussian(m, n, 3000 + i))
+            leads.append(None)
+        elif i % 3 == 1:
+            mats.append(inputs.graded(m, n, 4000 + i))
+            leads.append(None)
+        else:
+            r = int(rng.integers(10, 60))
+            mats.append(inputs.lowrank_noise(m, n, r, 5000 + i, noise=0.0))
+            leads.append(r)
+    plan = qrdm_amd.BatchPlan([a.shape for a in mats], nb=64)
+    plan.set_panel_rows(128)
+    say("tall batch of %d" % len(mats))
+    buf, tau, jp, nc, info = run_batch(gpu, plan, pack(torch, plan, mats))
+    for i, a in enumerate(mats):
+        got = unpack(plan, i, buf, tau, jp, nc, info)
+        ref = oracle.dgeqrdm(a)
+        try:
+            check(got, ref, leads[i])
+        except AssertionError:
+            say("matrix", i, a.shape, "lead", leads[i])
+            raise
 
 
 def test_batched_eight_wave_shape_is_refused(gpu):
@@ -229,8 +274,8 @@ def test_config4_batch_vs_plain_loop(gpu, oracle):
     """4096 uneven matrices (m in [64, 1024], n in [32, min(m, 512)], spectra
     log-uniform over [1e-10, 1]) in one launch, against a loop of the plain
     device path over the same matrices.  Both paths are rounding-determined in
-    the tails (see the module docstring), and the plain path's tall panels
-    are Gram-based while the batch uses the column-by-column panel, so the
+    the tails (see the module docstring), and both take the Gram-based panel
+    on tall panels but sum its products in different orders, so the
     check is: every matrix factorised (info 0), the leading pivots
     (sigma > 1e-6) identical on all but at most 8 of 4096 matrices (measured:
     2), the leading R blocks of identical-pivot matrices within 1e-12; each
