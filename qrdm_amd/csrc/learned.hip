@@ -303,9 +303,9 @@ __global__ __launch_bounds__(256) void k_policy_pack(const float* __restrict__ t
 // The label is the grid point with the largest such block (ties: the
 // defaults first, then larger tau, then smaller delta).  One wave; L is
 // KMAX * (KMAX + 1) doubles of LDS scratch.
-constexpr int LBL_NT = 4, LBL_ND = 4;
-__device__ constexpr double kLblTau[LBL_NT] = {0.25, 0.15, 0.10, 0.05};
-__device__ constexpr double kLblDelta[LBL_ND] = {0.80, 0.90, 0.95, 0.98};
+constexpr int LBL_NT = 5, LBL_ND = 6;
+__device__ constexpr double kLblTau[LBL_NT] = {0.25, 0.15, 0.10, 0.05, 0.01};
+__device__ constexpr double kLblDelta[LBL_ND] = {0.80, 0.90, 0.95, 0.98, 0.99, 0.999};
 
 __device__ int label_block(const double* G, const double* cand, int kmax, double tau,
                            double delta, double tau_def, double* L) {

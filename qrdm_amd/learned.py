@@ -29,7 +29,7 @@ import numpy as np
 
 from . import BatchPlan, lib, last_error
 
-RANGES = (0.02, 0.40, 0.75, 0.99)  # tau_lo, tau_hi, delta_lo, delta_hi
+RANGES = (0.01, 0.40, 0.75, 0.999)  # tau_lo, tau_hi, delta_lo, delta_hi
 FAMILIES = ("geometric", "step", "kahan")
 HERE = os.path.dirname(os.path.abspath(__file__))
 THETA_PATH = os.path.join(HERE, "policy_theta.npy")
@@ -222,6 +222,28 @@ def load_policy(torch, path=THETA_PATH):
     return pack(torch, np.load(path, allow_pickle=False))
 
 
+def label_stats(recs, families=FAMILIES):
+    """What the label grid can gain over the defaults under the panel's
+    truncation judge, from trace records (in family order, as
+    collect_traces writes them): the fraction of steps where some grid
+    point keeps a larger block, the mean blocks, and which of tau and delta
+    the winning labels move."""
+    tau = RANGES[0] + (RANGES[1] - RANGES[0]) * recs[:, 34].astype(np.float64)
+    delta = RANGES[2] + (RANGES[3] - RANGES[2]) * recs[:, 35].astype(np.float64)
+    d_blk, b_blk = recs[:, 36], recs[:, 37]
+    gain = b_blk > d_blk
+    out = dict(steps=int(len(recs)), frac_steps_improvable=float(gain.mean()),
+               mean_block_default=float(d_blk.mean()), mean_block_best=float(b_blk.mean()),
+               mean_gain_where_improvable=float((b_blk - d_blk)[gain].mean()) if gain.any() else 0.0,
+               winners_tau_below_default=int(np.sum(gain & (tau < 0.15 - 1e-6))),
+               winners_tau_above_default=int(np.sum(gain & (tau > 0.15 + 1e-6))),
+               winners_delta_above_default=int(np.sum(gain & (delta > 0.9 + 1e-6))),
+               winners_delta_below_default=int(np.sum(gain & (delta < 0.9 - 1e-6))))
+    vals, cnt = np.unique(np.round(delta[gain], 3), return_counts=True)
+    out["winning_delta"] = {str(v): int(c) for v, c in zip(vals, cnt)}
+    return out
+
+
 def evaluate(torch, policy_net, m=8192, n=4096, seed=777, reps=3, thres=(0.9, 0.15), log=None):
     """Default parameters against the learned proposal on one matrix of each
     family: DM steps, time per factorisation, backward error and the
@@ -282,8 +304,8 @@ def main(argv=None):
     import time
     import torch
     ap = argparse.ArgumentParser(prog="python -m qrdm_amd.learned")
-    ap.add_argument("cmd", choices=["train", "eval"])
-    ap.add_argument("--matrices", type=int, default=20000, help="training matrices in all")
+    ap.add_argument("cmd", choices=["train", "eval", "stats"])
+    ap.add_argument("--matrices", type=int, default=6000, help="training matrices in all")
     ap.add_argument("--m", type=int, default=2048)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=20)
@@ -300,6 +322,7 @@ def main(argv=None):
         recs = collect_traces(torch, args.matrices // len(FAMILIES), m=args.m, n=args.n,
                               seed=args.seed, log=log)
         t1 = time.time()
+        log(json.dumps(dict(label_stats=label_stats(recs))))
         theta, hist = train(torch, recs[:, :34], recs[:, 34:36], epochs=args.epochs,
                             seed=args.seed, log=log)
         t2 = time.time()
@@ -309,6 +332,10 @@ def main(argv=None):
                             heldout_loss=hist[-1][1],
                             default_block=float(recs[:, 36].mean()),
                             label_block=float(recs[:, 37].mean()), out=args.out)))
+    elif args.cmd == "stats":
+        recs = collect_traces(torch, args.matrices // len(FAMILIES), m=args.m, n=args.n,
+                              seed=args.seed, log=log)
+        log(json.dumps(label_stats(recs)))
     else:
         net = pack(torch, np.load(args.theta, allow_pickle=False))
         log(json.dumps(evaluate(torch, net, log=log)))

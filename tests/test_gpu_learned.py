@@ -195,6 +195,12 @@ def test_trace_records(gpu, oracle):
     from oracle import policy as pol
     torch = gpu
     mats = [inputs.graded(400, 160 + 40 * i, 60 + i) for i in range(3)]
+    # 20 orthonormal columns and 20 partners at cosine 0.93 with them (an
+    # orthogonal part of 0.37): delta 0.9 rejects the partners, delta >= 0.95
+    # keeps all 40 in the first block
+    q = np.linalg.qr(np.random.default_rng(5).standard_normal((400, 40)))[0]
+    mats.append(np.asfortranarray(np.hstack([q[:, :20], 0.99 * (0.93 * q[:, :20] + np.sqrt(
+        1 - 0.93 ** 2) * q[:, 20:])])))
     plan = qrdm_amd.BatchPlan([x.shape for x in mats], nb=64)
     buf = torch.zeros(plan.a_size, dtype=torch.float64, device="cuda")
     for i, x in enumerate(mats):
@@ -203,7 +209,7 @@ def test_trace_records(gpu, oracle):
     tau = torch.zeros(plan.tau_size, dtype=torch.float64, device="cuda")
     jp = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
     nc = torch.zeros(plan.col_size, dtype=torch.int32, device="cuda")
-    info = torch.zeros(3, dtype=torch.int32, device="cuda")
+    info = torch.zeros(len(mats), dtype=torch.int32, device="cuda")
     cap = 2000
     out = torch.zeros(cap * 40, dtype=torch.float32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -228,6 +234,11 @@ def test_trace_records(gpu, oracle):
         assert len(r0) == 1
         f0 = pol.features(np.linalg.norm(x, axis=0), 0, *x.shape)
         assert np.abs(r0[0, :34] - f0).max() <= 1e-6
+        # its blocks: the default's and the best grid point's
+        d0, b0 = _label_blocks(x)
+        assert (r0[0, 36], r0[0, 37]) == (d0, b0), (i, r0[0, 36:38], d0, b0)
+        if i == 3:
+            assert (d0, b0) == (20, 40)
     assert len(recs) == steps
     assert (recs[:, 34:36] >= -1e-6).all() and (recs[:, 34:36] <= 1 + 1e-6).all()
     assert (recs[:, 37] >= recs[:, 36]).all()
@@ -235,10 +246,48 @@ def test_trace_records(gpu, oracle):
         % (len(recs), recs[:, 37].sum() / recs[:, 36].sum()))
 
 
+LABEL_TAU = (0.25, 0.15, 0.10, 0.05, 0.01)
+LABEL_DELTA = (0.80, 0.90, 0.95, 0.98, 0.99, 0.999)
+
+
+def _label_blocks(x, tau_def=0.15, delta_def=0.9, kmax=64):
+    """The label scoring of a first DM step in numpy: the top-kmax column
+    norms (value descending, position ascending) are the candidates; a grid
+    point (tau, delta) keeps those above tau * max, accepts greedily in that
+    order each whose |cos| with every accepted one is below delta, and is
+    scored by the accepted columns' Cholesky factor cut at the first
+    diagonal below tau_def * r_00 (the panel's truncation test).  Returns
+    (the defaults' block, the largest block over the grid)."""
+    nrm = np.linalg.norm(x, axis=0)
+    order = sorted(range(len(nrm)), key=lambda c: (-nrm[c], c))[:min(kmax, *x.shape)]
+    cand = nrm[order]
+    G = x[:, order].T @ x[:, order]
+
+    def block(tau, delta):
+        nc = int(np.sum(cand > tau * cand[0]))
+        acc = [0]
+        for i in range(1, nc):
+            if all(abs(G[q, i] / cand[q] / cand[i]) < delta for q in acc):
+                acc.append(i)
+        L = np.zeros((len(acc), len(acc)))
+        for k in range(len(acc)):
+            d = G[acc[k], acc[k]] - L[k, :k] @ L[k, :k]
+            L[k, k] = np.sqrt(max(d, 0.0))
+            if k > 0 and not L[k, k] >= tau_def * L[0, 0]:
+                return k
+            for t in range(k + 1, len(acc)):
+                L[t, k] = (G[acc[t], acc[k]] - L[t, :k] @ L[k, :k]) / L[k, k] if L[k, k] > 0 else 0.0
+        return len(acc)
+
+    d = block(tau_def, delta_def)
+    return d, max([d] + [block(t, dl) for t in LABEL_TAU for dl in LABEL_DELTA])
+
+
 @pytest.mark.slow
 def test_trained_policy_against_defaults(gpu):
-    """The committed network (trained on the device on 20000 seeded 2048 x
-    1024 matrices of the three families, python -m qrdm_amd.learned train)
+    """The committed network (trained on the device on 6000 seeded 2048 x
+    1024 matrices of the three families, python -m qrdm_amd.learned train
+    --matrices 6000)
     against the default parameters on one 8192 x 4096 matrix per family: no
     loss in backward error or in the rank-revealing ratio min_k |R_kk| /
     sigma_k (within a factor 2 of the defaults'), no more than one extra DM
