@@ -462,31 +462,49 @@ __global__ __launch_bounds__(256) void k_wsum_norm(double* A, int lda, int m, in
   lds_barrier();
   unsigned long long rows = 0;
   if (s_any) {
-    // the recomputed columns take the update on rows [jn, m) now
-    double ps[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int rs = jn; rs < m; rs += KMAX) {
-      for (int q = 0; q < KMAX * KMAX / 256; q++) {
-        const int e = tid + 256 * q, r = e & 63, cc = e >> 6;
-        const bool rin = rs + r < m;
-        S0[r][cc] = rin ? V[(size_t)cc * ldv + rs + r] : 0.0;
-        S2[r][cc] = (rin && s_rec[cc]) ? A[(size_t)(cb + cc) * lda + rs + r] : 0.0;
-      }
-      lds_barrier();
-      mfma_64(acc, S0, false, S1);
+    // the recomputed columns take the update on rows [jn, m) now.  Each wave
+    // runs its own 16-row blocks (rows jn + 16 wv, then every 64th): D =
+    // V_blk W on the matrix cores, V read straight into the A operand, W from
+    // LDS, no workgroup barrier inside the loop.  Lane (g, li) keeps the sums
+    // of squares of columns 16 u + li over its rows g + 4 v of each block.
+    bool fl[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int cc = 16 * u + li;
+    for (int u = 0; u < 4; u++) fl[u] = s_rec[16 * u + li] != 0;
+    double ps[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int rb = jn + 16 * wv; rb < m; rb += 4 * 16) {
+      const int ra = rb + li;  // the lane's row of the A operand
+      double va[KMAX / 4];
+#pragma unroll
+      for (int t = 0; t < KMAX / 4; t++)
+        va[t] = (ra < m) ? V[(size_t)(4 * t + g) * ldv + ra] : 0.0;  // V(ra, 4t + g)
+      double cv[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int r = 16 * wv + g + 4 * v;
-          if (s_rec[cc] && rs + r < m) {
-            const double x = S2[r][cc] - acc[u][v];
-            A[(size_t)(cb + cc) * lda + rs + r] = x;
+          const int r = rb + g + 4 * v;
+          cv[u][v] = (fl[u] && r < m) ? A[(size_t)(cb + 16 * u + li) * lda + r] : 0.0;
+        }
+      dbl4 d[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) d[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < KMAX / 4; t++)
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          d[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[t], S1[4 * t + g][16 * u + li], d[u], 0,
+                                                      0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int r = rb + g + 4 * v;
+          if (fl[u] && r < m) {
+            const double x = cv[u][v] - d[u][v];
+            A[(size_t)(cb + 16 * u + li) * lda + r] = x;
             ps[u] = fma(x, x, ps[u]);
           }
         }
-      }
-      lds_barrier();
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
