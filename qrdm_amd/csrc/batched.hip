@@ -234,13 +234,50 @@ __device__ __forceinline__ void bt_gram(int r0, int m, int ncols, ColFn colval, 
   __syncthreads();
 }
 
-// S += the 64 x 64 product mfma_64 left in acc (each wave adds its own rows).
-__device__ __forceinline__ void bt_add_tile(double (*S)[LDP], const dbl4p (&acc)[4]) {
-  const int t = local_tid(), lane = t & 63, wv = t >> 6;
+// One 16-row block's contribution to a wave's Gram X^T X (64 x 64): xb[u]
+// holds X_blk(:, 16u .. 16u + 15) in the matrix cores' output layout (rows
+// g + 4v, column 16u + li for lane 16g + li), which is also the operand layout
+// of a product over those rows, so the tiles u <= u2 accumulate with no data
+// movement.
+__device__ __forceinline__ void bt_gram_tiles(const dbl4p (&xb)[4], dbl4p (&gt)[10]) {
+  int p = 0;
 #pragma unroll
   for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int v = 0; v < 4; v++) S[16 * wv + (lane >> 4) + 4 * v][16 * u + (lane & 15)] += acc[u][v];
+    for (int u2 = u; u2 < 4; u2++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        gt[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[u][v], xb[u2][v], gt[p], 0, 0, 0);
+      p++;
+    }
+}
+
+// S = the sum of the four waves' tile Grams gt (in wave order), both
+// triangles; four waves.  Ends with a barrier.
+__device__ __forceinline__ void bt_gram_store(double (*S)[LDP], const dbl4p (&gt)[10]) {
+  const int t = local_tid(), lane = t & 63, wv = t >> 6, g = lane >> 4, li = lane & 15;
+  for (int e = t; e < KMAX * KMAX; e += 256) S[e >> 6][e & 63] = 0.0;
+  lds_barrier();
+  for (int w = 0; w < 4; w++) {
+    if (wv == w) {
+      int p = 0;
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int u2 = u; u2 < 4; u2++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) S[16 * u + g + 4 * v][16 * u2 + li] += gt[p][v];
+          p++;
+        }
+    }
+    lds_barrier();
+  }
+  // the diagonal tiles are whole; mirror the tiles below them
+  for (int e = t; e < KMAX * KMAX; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if ((r >> 4) > (c >> 4)) S[r][c] = S[c][r];
+  }
+  lds_barrier();
 }
 
 // The Gram-based panel (panel.hip's tall-panel form) by the workgroup: R1 =
@@ -282,21 +319,33 @@ __device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, i
     }
   }
   lds_barrier();
-  // 2. G2 = (P R1^-1)^T (P R1^-1), slab by slab through TS, summed into RT
-  // (each wave adds its own rows); identity beyond k1
-  dbl4p acc[4];
-  for (int e = tid; e < KMAX * KMAX; e += NW * 64) RT[e >> 6][e & 63] = 0.0;
-  for (int rs = j; rs < m; rs += PR_ROWS) {
-    load_slab(TS, a, lda, j, rs, m, k1);
-    lds_barrier();
-    mfma_64(acc, TS, false, E1);
-    lds_barrier();
-    mfma_store(TS, acc);
-    lds_barrier();
-    mfma_64(acc, TS, true, TS);
-    bt_add_tile(RT, acc);
-    lds_barrier();  // the next slab overwrites TS
+  // 2. G2 = (P R1^-1)^T (P R1^-1): each wave takes its own 16-row blocks (rows
+  // j + 16 wv, then every 64th) with no barrier: Q_blk = P_blk R1^-1 on the
+  // matrix cores, P read straight into the A operand; Q_blk^T Q_blk from the
+  // product's own layout (bt_gram_tiles).  The waves' sums go into RT in wave
+  // order; identity beyond k1.
+  const int g = lane >> 4, li = lane & 15;
+  dbl4p gt[10];
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+  for (int rb = j + 16 * wv; rb < m; rb += NW * 16) {
+    const int ra = rb + li;
+    double pa[KMAX / 4];
+#pragma unroll
+    for (int t = 0; t < KMAX / 4; t++)
+      pa[t] = (ra < m && 4 * t + g < k1) ? a[(size_t)(j + 4 * t + g) * lda + ra] : 0.0;
+    dbl4p qb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) qb[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < KMAX / 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        qb[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t], E1[4 * t + g][16 * u + li], qb[u], 0, 0,
+                                                     0);
+    bt_gram_tiles(qb, gt);
   }
+  bt_gram_store(RT, gt);
   if (tid >= k1 && tid < KMAX) RT[tid][tid] = 1.0;
   lds_barrier();
   // 3. the reconstruction: SR = E0, SA = E1 (R1^-1), SW = TS (also its
@@ -308,36 +357,42 @@ __device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, i
   __syncthreads();  // the top block the reconstruction wrote to A is read below
   for (int e = tid; e < KMAX * KMAX; e += NW * 64) E0[e & 63][e >> 6] = Mf[e];  // M, padded
   lds_barrier();
-  // 4. V2 = P_bottom M, slab by slab through RT, written into the panel below
-  // the top block (which keeps V1); V^T V summed into E1
-  for (int e = tid; e < KMAX * KMAX; e += NW * 64) E1[e >> 6][e & 63] = 0.0;
-  for (int rs = j; rs < m; rs += PR_ROWS) {
-    load_slab(RT, a, lda, j, rs, m, k);
-    lds_barrier();
-    mfma_64(acc, RT, false, E0);
-    lds_barrier();
+  // 4. V2 = P_bottom M: each wave its own 16-row blocks as in 2, P straight
+  // into the A operand, M (E0) the B operand; the top block's rows take V1
+  // (unit lower, written into A by the reconstruction), V2's rows are
+  // written into the panel, and V^T V accumulates from the product's layout
+  // (summed into E1 in wave order).
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+  for (int rb = j + 16 * wv; rb < m; rb += NW * 16) {
+    const int ra = rb + li;
+    double pa[KMAX / 4];
+#pragma unroll
+    for (int t = 0; t < KMAX / 4; t++)
+      pa[t] = (ra < m && 4 * t + g < k) ? a[(size_t)(j + 4 * t + g) * lda + ra] : 0.0;
+    dbl4p vb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) vb[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < KMAX / 4; t++)
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        vb[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t], E0[4 * t + g][16 * u + li], vb[u], 0, 0,
+                                                     0);
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-        const int pr = rs - j + r;
-        double x = acc[u][v];
-        if (c >= k) x = 0.0;
-        else if (pr < k) x = (pr < c) ? 0.0 : (pr == c ? 1.0 : RT[r][c]);
-        acc[u][v] = x;
+        const int r = rb + g + 4 * v, c = 16 * u + li, pr = r - j;
+        double x = vb[u][v];
+        if (c >= k || r >= m) x = 0.0;
+        else if (pr < k) x = (pr < c) ? 0.0 : (pr == c ? 1.0 : a[(size_t)(j + c) * lda + r]);
+        else a[(size_t)(j + c) * lda + r] = x;
+        vb[u][v] = x;
       }
-    lds_barrier();
-    mfma_store(RT, acc);
-    lds_barrier();
-    for (int e = tid; e < KMAX * PR_ROWS; e += NW * 64) {
-      const int c = e / PR_ROWS, r = e % PR_ROWS, row = rs + r;
-      if (row < m && c < k && row - j >= k) a[(size_t)(j + c) * lda + row] = RT[r][c];
-    }
-    mfma_64(acc, RT, true, RT);
-    bt_add_tile(E1, acc);
-    lds_barrier();  // the next slab overwrites RT
+    bt_gram_tiles(vb, gt);
   }
+  bt_gram_store(E1, gt);
   // 5. tau_i = 2 / S_ii and T = (striu(S) + diag(S) / 2)^-1 from S = V^T V
   for (int e = tid; e < KMAX * KMAX; e += NW * 64) {
     const int r = e & 63, c = e >> 6;
