@@ -195,45 +195,6 @@ __device__ __forceinline__ void bt_trailing(double* a, int lda, int j, int m, in
   }
 }
 
-// Gram X^T X of up to 64 columns over rows [r0, m), X(r, c) = colval(c, r),
-// into R1 (leading dimension KMAX + 1); waves 0..3 run the matrix cores,
-// every wave loads.
-template <int NW, typename ColFn>
-__device__ __forceinline__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L) {
-  constexpr int NT = 64 * NW;
-  double* X = L.R1;
-  const int tid = local_tid(), wave = tid >> 6;
-  dbl4 acc[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
-  for (int rs = r0; rs < m; rs += GRAM_ROWS) {
-    double xv[KMAX * GRAM_ROWS / NT];
-#pragma unroll
-    for (int q = 0; q < KMAX * GRAM_ROWS / NT; q++) {
-      const int e = tid + NT * q, c = e / GRAM_ROWS, r = rs + e % GRAM_ROWS;
-      xv[q] = (c < ncols && r < m) ? colval(c, r) : 0.0;
-    }
-    __syncthreads();  // the previous slab's products are done with X
-#pragma unroll
-    for (int q = 0; q < KMAX * GRAM_ROWS / NT; q++) {
-      const int e = tid + NT * q;
-      X[(e / GRAM_ROWS) * (GRAM_ROWS + 1) + e % GRAM_ROWS] = xv[q];
-    }
-    __syncthreads();
-    if (wave < 4) gram_slab_acc(X, acc);
-  }
-  __syncthreads();
-  if (wave < 4) {
-    const int lane = tid & 63, g = lane >> 4, li = lane & 15;
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        X[(16 * wave + g + 4 * v) + (16 * u + li) * (KMAX + 1)] = acc[u][v];
-  }
-  __syncthreads();
-}
-
 // One 16-row block's contribution to a wave's Gram X^T X (64 x 64): xb[u]
 // holds X_blk(:, 16u .. 16u + 15) in the matrix cores' output layout (rows
 // g + 4v, column 16u + li for lane 16g + li), which is also the operand layout
@@ -278,6 +239,33 @@ __device__ __forceinline__ void bt_gram_store(double (*S)[LDP], const dbl4p (&gt
     if ((r >> 4) > (c >> 4)) S[r][c] = S[c][r];
   }
   lds_barrier();
+}
+
+// Gram X^T X of up to 64 columns over rows [r0, m), X(r, c) = colval(c, r),
+// into R1 (both triangles, leading dimension KMAX + 1); four waves.  Each
+// wave takes its own 16-row blocks (rows r0 + 16 wv, then every 64th), reads
+// them straight into the matrix cores' operand layout and accumulates the
+// upper tiles in registers (bt_gram_tiles); no barrier until the waves' sums
+// go into R1 in wave order.
+template <int NW, typename ColFn>
+__device__ __forceinline__ void bt_gram(int r0, int m, int ncols, ColFn colval, BatchLds<NW>& L) {
+  static_assert(NW == 4, "the Gram's tiles are spread over four waves");
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6, g = lane >> 4, li = lane & 15;
+  dbl4p gt[10];
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+  for (int rb = r0 + 16 * wv; rb < m; rb += NW * 16) {
+    dbl4p xb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int c = 16 * u + li, r = rb + g + 4 * v;
+        xb[u][v] = (c < ncols && r < m) ? colval(c, r) : 0.0;
+      }
+    bt_gram_tiles(xb, gt);
+  }
+  bt_gram_store(reinterpret_cast<double (*)[LDP]>(L.R1), gt);
 }
 
 // The Gram-based panel (panel.hip's tall-panel form) by the workgroup: R1 =
