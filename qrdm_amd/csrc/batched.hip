@@ -3,13 +3,17 @@
 // taking the next one (in decreasing flop count, ordered by the host) from a
 // device counter, with the DM state of the current matrix in LDS.
 //
-// The arithmetic of a step is the plain path's (src/dgeqrdm_work.c:694-787)
-// with the column-by-column panel: the same device routines for the norms
-// (wave_nrm2), the top-k order (wave_sort64 / wave_merge64), the greedy
-// acceptance and permute_marked plan (select_wave), dgeqr2_mia
-// (panel_exact_body) and dlarft (larft_recurrence); the candidate Gram, V^T V
-// and the trailing update run on the FP64 matrix cores inside the workgroup.
-// Limits: n <= BT_NMAX (the per-column state lives in LDS); m is free.
+// The arithmetic of a step is the plain path's (src/dgeqrdm_work.c:694-787):
+// the same device routines for the norms (wave_nrm2), the top-k order
+// (wave_sort64 / wave_merge64), the greedy acceptance and permute_marked
+// plan (select_wave), and the panel in either of the plain path's forms --
+// on panels of at least gram_rows rows the Gram-based one (chol1_core,
+// recon_core: CholeskyQR2 with Householder reconstruction, T from V^T V),
+// otherwise dgeqr2_mia column by column (panel_exact_body) with dlarft
+// (larft_recurrence).  The candidate Gram, the panel's products and Grams
+// and the trailing update run on the FP64 matrix cores inside the
+// workgroup.  Limits: n <= BT_NMAX (the per-column state lives in LDS); m
+// is free.
 
 namespace qrdm {
 
