@@ -199,24 +199,6 @@ __device__ __forceinline__ void bt_trailing(double* a, int lda, int j, int m, in
   }
 }
 
-// One 16-row block's contribution to a wave's Gram X^T X (64 x 64): xb[u]
-// holds X_blk(:, 16u .. 16u + 15) in the matrix cores' output layout (rows
-// g + 4v, column 16u + li for lane 16g + li), which is also the operand layout
-// of a product over those rows, so the tiles u <= u2 accumulate with no data
-// movement.
-__device__ __forceinline__ void bt_gram_tiles(const dbl4p (&xb)[4], dbl4p (&gt)[10]) {
-  int p = 0;
-#pragma unroll
-  for (int u = 0; u < 4; u++)
-#pragma unroll
-    for (int u2 = u; u2 < 4; u2++) {
-#pragma unroll
-      for (int v = 0; v < 4; v++)
-        gt[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[u][v], xb[u2][v], gt[p], 0, 0, 0);
-      p++;
-    }
-}
-
 // S = the sum of the four waves' tile Grams gt (in wave order), both
 // triangles; four waves.  Ends with a barrier.
 __device__ __forceinline__ void bt_gram_store(double (*S)[LDP], const dbl4p (&gt)[10]) {
@@ -267,7 +249,7 @@ __device__ __forceinline__ void bt_gram(int r0, int m, int ncols, ColFn colval, 
         const int c = 16 * u + li, r = rb + g + 4 * v;
         xb[u][v] = (c < ncols && r < m) ? colval(c, r) : 0.0;
       }
-    bt_gram_tiles(xb, gt);
+    gram_tiles(xb, gt);
   }
   bt_gram_store(reinterpret_cast<double (*)[LDP]>(L.R1), gt);
 }
@@ -335,7 +317,7 @@ __device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, i
       for (int u = 0; u < 4; u++)
         qb[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t], E1[4 * t + g][16 * u + li], qb[u], 0, 0,
                                                      0);
-    bt_gram_tiles(qb, gt);
+    gram_tiles(qb, gt);
   }
   bt_gram_store(RT, gt);
   if (tid >= k1 && tid < KMAX) RT[tid][tid] = 1.0;
@@ -382,7 +364,7 @@ __device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, i
         else a[(size_t)(j + c) * lda + r] = x;
         vb[u][v] = x;
       }
-    bt_gram_tiles(vb, gt);
+    gram_tiles(vb, gt);
   }
   bt_gram_store(E1, gt);
   // 5. tau_i = 2 / S_ii and T = (striu(S) + diag(S) / 2)^-1 from S = V^T V

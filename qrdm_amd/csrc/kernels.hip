@@ -503,6 +503,24 @@ __device__ __forceinline__ void gram_store(const dbl4 (&acc)[4], double* out) {
     for (int v = 0; v < 4; v++) out[(16 * wv + g + 4 * v) + (16 * u + li) * KMAX] = acc[u][v];
 }
 
+// One 16-row block's contribution to a wave's Gram X^T X (64 x 64): xb[u]
+// holds X_blk(:, 16u .. 16u + 15) in the matrix cores' output layout (rows
+// g + 4v, column 16u + li for lane 16g + li), which is also the operand layout
+// of a product over those rows, so the tiles u <= u2 accumulate with no data
+// movement.
+__device__ __forceinline__ void gram_tiles(const dbl4 (&xb)[4], dbl4 (&gt)[10]) {
+  int p = 0;
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int u2 = u; u2 < 4; u2++) {
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        gt[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(xb[u][v], xb[u2][v], gt[p], 0, 0, 0);
+      p++;
+    }
+}
+
 __device__ __forceinline__ void gram_slab_mfma(const double* X, double* out) {
   dbl4 acc[4];
 #pragma unroll
@@ -730,87 +748,97 @@ __device__ __forceinline__ void select_wave(const double* Gs, double delta, int 
 
 // The DM step's candidate Gram over rows j..m (cblas_dsyrk,
 // src/dgeqrdm_work.c:376-379) as per-slab partials, one 64-row slab per
-// workgroup.  With a pending update (the previous step's, deferred: st->fused)
+// workgroup and a 16-row block per wave, read straight into the matrix cores'
+// layout (the Gram's tiles accumulate from it, gram_tiles; only the upper
+// triangle of a partial is written, all the reduce reads).  With a pending
+// update (the previous step's, deferred: st->fused)
 // the candidate columns first receive it on the slab, X -= V_prev(slab rows)
 // W_prev(:, candidates) on the matrix cores, and the updated rows are written
 // back: the cosines, the panel and the later steps read updated columns.
 // Slabs past m (the grid is sized for the lowest j) write zero partials.
 __global__ __launch_bounds__(256) void k_gram_cand(double* A, int lda, int m, double* partial,
                                                    const DevState* st, StepBufs bufs) {
-  __shared__ double X[KMAX * (GRAM_ROWS + 1)];
-  __shared__ double Vs[KMAX * (GRAM_ROWS + 1)];
-  __shared__ double Ws[KMAX * (KMAX + 1)];
+  __shared__ double Ws[KMAX][KMAX + 1];       // W_prev(i, candidate c) at Ws[i][c]
+  __shared__ double Gp[4][10][4][64];         // each wave's tiles, in its lanes' layout
   if (dm_stopped(st)) return;
   const int j = st->j;
   // the greedy takes candidate 0 even when nc is 0 (all norms zero), so it
   // receives the update too; the cosines read the Gram only when nc > 1
   const int nc = max(st->nc, 1);
-  const int rs = j + blockIdx.x * GRAM_ROWS;
-  const int tid = threadIdx.x;
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6, g = lane >> 4, li = lane & 15;
+  // wave wv takes rows rb .. rb + 15 of the workgroup's 64-row slab, read
+  // straight into the matrix cores' layout: X(rb + g + 4v, candidate 16u + li)
+  const int rb = j + blockIdx.x * GRAM_ROWS + 16 * wv;
   const bool pend = update_pending(st);
-  constexpr int Q = KMAX * GRAM_ROWS / 256;
-  double xv[Q];
+  double* cc[4];
+  bool cok[4];
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int t = tid + 256 * q, c = t / GRAM_ROWS, r = t % GRAM_ROWS;
-    const bool ok = c < nc && rs + r < m;
-    xv[q] = ok ? A[(size_t)(j + st->cand_idx[c]) * lda + rs + r] : 0.0;
+  for (int u = 0; u < 4; u++) {
+    const int c = 16 * u + li;
+    cok[u] = c < nc;
+    cc[u] = A + (size_t)(j + (cok[u] ? st->cand_idx[c] : 0)) * lda;
   }
+  dbl4 xb[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int r = rb + g + 4 * v;
+      xb[u][v] = (cok[u] && r < m) ? cc[u][r] : 0.0;
+    }
   if (pend) {
+    // X -= V_prev(rows, :) W_prev(:, candidates) on the matrix cores: V rows
+    // straight into the A operand, W through LDS; the product comes out in
+    // X's layout.  The updated rows are written back.
     const double* Vp = v_prev(bufs, st);
     const double* Wq = w_prev(bufs, st);
-    double vv[Q], wv[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = tid + 256 * q, i = t / GRAM_ROWS, r = t % GRAM_ROWS;
-      vv[q] = (rs + r < m) ? Vp[(size_t)i * bufs.ldv + rs + r] : 0.0;
-      // W(i2, candidate c): a candidate's 64 entries are contiguous
-      const int i2 = t % KMAX, c = t / KMAX;
-      wv[q] = (c < nc) ? Wq[(size_t)(j + st->cand_idx[c]) * KMAX + i2] : 0.0;
+    for (int e = tid; e < KMAX * KMAX; e += 256) {
+      const int i = e & 63, c = e >> 6;  // a candidate's 64 entries are contiguous
+      Ws[i][c] = (c < nc) ? Wq[(size_t)(j + st->cand_idx[c]) * KMAX + i] : 0.0;
     }
+    double va[KMAX / 4];
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = tid + 256 * q;
-      X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
-      Vs[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = vv[q];
-      Ws[(t / KMAX) * (KMAX + 1) + t % KMAX] = wv[q];
-    }
+    for (int t = 0; t < KMAX / 4; t++)
+      va[t] = (rb + li < m) ? Vp[(size_t)(4 * t + g) * bufs.ldv + rb + li] : 0.0;
     __syncthreads();
-    // D(r, c) = sum_i V_prev(rs + r, i) W_prev(i, c): wave w rows 16w..16w+15
-    const int lane = tid & 63, wv_ = tid >> 6, g = lane >> 4, li = lane & 15;
-    dbl4 acc[4];
+    dbl4 d[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) acc[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int k0 = 0; k0 < KMAX; k0 += 4) {
-      const double a = Vs[(k0 + g) * (GRAM_ROWS + 1) + 16 * wv_ + li];
+    for (int u = 0; u < 4; u++) d[u] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < KMAX / 4; t++)
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ws[(16 * u + li) * (KMAX + 1) + k0 + g],
-                                                      acc[u], 0, 0, 0);
-    }
-    // every (row, column) of X belongs to one lane: no barrier before the update
+        d[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[t], Ws[4 * t + g][16 * u + li], d[u], 0, 0,
+                                                    0);
 #pragma unroll
     for (int u = 0; u < 4; u++)
 #pragma unroll
-      for (int v = 0; v < 4; v++)
-        X[(16 * u + li) * (GRAM_ROWS + 1) + 16 * wv_ + g + 4 * v] -= acc[u][v];
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = tid + 256 * q, c = t / GRAM_ROWS, r = t % GRAM_ROWS;
-      if (c < nc && rs + r < m)
-        A[(size_t)(j + st->cand_idx[c]) * lda + rs + r] = X[c * (GRAM_ROWS + 1) + r];
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const int t = tid + 256 * q;
-      X[(t / GRAM_ROWS) * (GRAM_ROWS + 1) + t % GRAM_ROWS] = xv[q];
-    }
-    __syncthreads();
+      for (int v = 0; v < 4; v++) {
+        const int r = rb + g + 4 * v;
+        xb[u][v] -= d[u][v];
+        if (cok[u] && r < m) cc[u][r] = xb[u][v];
+      }
   }
-  gram_slab_mfma(X, partial + (size_t)blockIdx.x * KMAX * KMAX);
+  // the slab's Gram: each wave's upper tiles from X's layout, then their sum
+  // over the waves in a fixed order.  The reduce reads the upper triangle
+  // only, so the tiles below the diagonal blocks are not written.
+  dbl4 gt[10];
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4){0.0, 0.0, 0.0, 0.0};
+  gram_tiles(xb, gt);
+#pragma unroll
+  for (int p = 0; p < 10; p++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) Gp[wv][p][v][lane] = gt[p][v];
+  __syncthreads();
+  double* out = partial + (size_t)blockIdx.x * KMAX * KMAX;
+  for (int idx = tid; idx < 10 * 4 * 64; idx += 256) {
+    const int p = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
+    const int u = p < 4 ? 0 : (p < 7 ? 1 : (p < 9 ? 2 : 3));
+    const int u2 = u + p - (u == 0 ? 0 : (u == 1 ? 4 : (u == 2 ? 7 : 9)));
+    const double s = ((Gp[0][p][v][ln] + Gp[1][p][v][ln]) + Gp[2][p][v][ln]) + Gp[3][p][v][ln];
+    out[(16 * u + (ln >> 4) + 4 * v) + (size_t)(16 * u2 + (ln & 15)) * KMAX] = s;
+  }
 }
 
 // The DM step's candidate Gram reduce (as k_gram_reduce, RUN_DM), then the
