@@ -641,34 +641,62 @@ __device__ __forceinline__ void load_mat(double (*S)[LDP], const double* __restr
   }
 }
 
-// Pass 2: Q = P R1^-1 on a 64-row slab, partial Gram Q^T Q.
-// Slabs past m (the grid is sized for the lowest j) write zero partials.
+// The sum over the four waves (in wave order) of their upper-tile Grams gt,
+// both triangles, column-major into out (64 x 64).  Gp: LDS for the waves'
+// tiles.  Ends after the writes (no barrier).
+__device__ __forceinline__ void gram_sum_store(const dbl4p (&gt)[10], double (*Gp)[10][4][64],
+                                               double* out) {
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int p = 0; p < 10; p++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) Gp[wv][p][v][lane] = gt[p][v];
+  lds_barrier();
+  for (int idx = tid; idx < 10 * 4 * 64; idx += 256) {
+    const int p = idx >> 8, v = (idx >> 6) & 3, ln = idx & 63;
+    const int u = p < 4 ? 0 : (p < 7 ? 1 : (p < 9 ? 2 : 3));
+    const int u2 = u + p - (u == 0 ? 0 : (u == 1 ? 4 : (u == 2 ? 7 : 9)));
+    const double x = ((Gp[0][p][v][ln] + Gp[1][p][v][ln]) + Gp[2][p][v][ln]) + Gp[3][p][v][ln];
+    const int r = 16 * u + (ln >> 4) + 4 * v, c = 16 * u2 + (ln & 15);
+    out[r + (size_t)c * KMAX] = x;
+    if (u != u2) out[c + (size_t)r * KMAX] = x;
+  }
+}
+
+// Pass 2: Q = P R1^-1 on a 64-row slab, partial Gram Q^T Q.  Wave w takes
+// rows 16w .. 16w + 15 of the slab, read straight into the matrix cores' A
+// operand; the product's output layout is the operand layout of Q^T Q over
+// those rows (gram_tiles), so nothing but R1^-1 goes through LDS.  Slabs
+// past m (the grid is sized for the lowest j) write zero partials.
 __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int lda, int m,
                                                const double* __restrict__ R1inv,
                                                const DevState* st, double* partial) {
-  __shared__ double Ps[PR_ROWS][LDP];  // P slab (row, col), then Q
   __shared__ double Ri[KMAX][LDP];
+  __shared__ double Gp[4][10][4][64];
   if (!step_runs(st, RUN_FAST)) return;
   const int j = st->j;
   const int k1 = st->k1;
-  const int tid = threadIdx.x;
-  const int rs = j + blockIdx.x * PR_ROWS;
-  load_slab(Ps, A, lda, j, rs, m, k1);
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6, g = lane >> 4, li = lane & 15;
+  const int rb = j + blockIdx.x * PR_ROWS + 16 * wv;
   load_mat(Ri, R1inv);
-  lds_barrier();
-  dbl4p acc[4];
-  mfma_64(acc, Ps, false, Ri);
-  lds_barrier();
-  mfma_store(Ps, acc);
-  lds_barrier();
-  mfma_64(acc, Ps, true, Ps);  // Q^T Q
-  const int lane = tid & 63, wv = tid >> 6;
-  double* out = partial + (size_t)blockIdx.x * KMAX * KMAX;
+  double pa[KMAX / 4];
 #pragma unroll
-  for (int u = 0; u < 4; u++)
+  for (int t = 0; t < KMAX / 4; t++)
+    pa[t] = (rb + li < m && 4 * t + g < k1) ? A[(size_t)(j + 4 * t + g) * lda + rb + li] : 0.0;
+  lds_barrier();
+  dbl4p qb[4];
 #pragma unroll
-    for (int v = 0; v < 4; v++)
-      out[(16 * wv + (lane >> 4) + 4 * v) + (16 * u + (lane & 15)) * KMAX] = acc[u][v];
+  for (int u = 0; u < 4; u++) qb[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < KMAX / 4; t++)
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      qb[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t], Ri[4 * t + g][16 * u + li], qb[u], 0, 0, 0);
+  dbl4p gt[10];
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+  gram_tiles(qb, gt);
+  gram_sum_store(gt, Gp, partial + (size_t)blockIdx.x * KMAX * KMAX);
 }
 
 // ---------------------------------------------------------------------------
@@ -981,18 +1009,20 @@ __global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__
 // ---------------------------------------------------------------------------
 // Pass 3: V2 = P_bottom M on a 64-row slab; writes V2 into the panel below the
 // top block and the dense zero-padded V (rows [j, m), KMAX columns) with V1's
-// unit lower block on top.
+// unit lower block on top, and the slab's V^T V.  As in pass 2, wave w takes
+// rows 16w .. 16w + 15 straight into the A operand (M through LDS), and V^T V
+// accumulates from the product's layout.
 __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
                                             const double* __restrict__ M, const DevState* st,
                                             StepBufs bufs, double* gram_part) {
-  __shared__ double Ps[PR_ROWS][LDP];
   __shared__ double Ms[KMAX][LDP];
+  __shared__ double Gp[4][10][4][64];
   if (!step_runs(st, RUN_FAST)) return;
   double* V = v_cur(bufs, st);
   const int ldv = bufs.ldv;
   const int j = st->j;
   const int k = st->fjb_cmp;
-  const int tid = threadIdx.x;
+  const int tid = local_tid(), lane = tid & 63, wv = tid >> 6, g = lane >> 4, li = lane & 15;
   const int rs = j + blockIdx.x * PR_ROWS;
   if (rs >= m) return;
   if (blockIdx.x == 0) {
@@ -1004,48 +1034,43 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
       if (r < j) V[(size_t)c * ldv + r] = 0.0;
     }
   }
-  load_slab(Ps, A, lda, j, rs, m, k);
+  const int rb = rs + 16 * wv;
   load_mat(Ms, M);
-  __syncthreads();
-  dbl4p acc[4];
-  mfma_64(acc, Ps, false, Ms);
-  __syncthreads();
-  // rows of the top block keep V1 (unit lower, written by k_recon): overlay
-  {
-    const int lane = tid & 63, wv = tid >> 6;
+  double pa[KMAX / 4];
+#pragma unroll
+  for (int t = 0; t < KMAX / 4; t++)
+    pa[t] = (rb + li < m && 4 * t + g < k) ? A[(size_t)(j + 4 * t + g) * lda + rb + li] : 0.0;
+  lds_barrier();
+  dbl4p vb[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) vb[u] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < KMAX / 4; t++)
 #pragma unroll
     for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int r = 16 * wv + (lane >> 4) + 4 * v, c = 16 * u + (lane & 15);
-        const int pr = rs - j + r;
-        double x = acc[u][v];
-        if (c >= k) x = 0.0;
-        else if (pr < k) x = (pr < c) ? 0.0 : (pr == c ? 1.0 : Ps[r][c]);
-        acc[u][v] = x;
-      }
-  }
-  __syncthreads();
-  mfma_store(Ps, acc);
-  __syncthreads();
-  for (int e = tid; e < KMAX * PR_ROWS; e += blockDim.x) {
-    const int c = e / PR_ROWS, r = e % PR_ROWS;
-    const int row = rs + r;
-    if (row < m) {
-      const double x = Ps[r][c];
-      V[(size_t)c * ldv + row] = x;
-      if (c < k && row - j >= k) A[(size_t)(j + c) * lda + row] = x;
-    }
-  }
-  // the slab's part of V^T V (rows past m are zero in Ps), for T and tau
-  mfma_64(acc, Ps, true, Ps);
-  const int lane = tid & 63, wv = tid >> 6;
-  double* out = gram_part + (size_t)blockIdx.x * KMAX * KMAX;
+      vb[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[t], Ms[4 * t + g][16 * u + li], vb[u], 0, 0, 0);
+  // rows of the top block keep V1 (unit lower, written by the
+  // reconstruction): overlay; then V and the panel's V2 rows out
 #pragma unroll
   for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int v = 0; v < 4; v++)
-      out[(16 * wv + (lane >> 4) + 4 * v) + (16 * u + (lane & 15)) * KMAX] = acc[u][v];
+    for (int v = 0; v < 4; v++) {
+      const int r = rb + g + 4 * v, c = 16 * u + li, pr = r - j;
+      double x = vb[u][v];
+      if (c >= k || r >= m) x = 0.0;
+      else if (pr < k) x = (pr < c) ? 0.0 : (pr == c ? 1.0 : A[(size_t)(j + c) * lda + r]);
+      vb[u][v] = x;
+      if (r < m) {
+        V[(size_t)c * ldv + r] = x;
+        if (c < k && pr >= k) A[(size_t)(j + c) * lda + r] = x;
+      }
+    }
+  // the slab's part of V^T V, for T and tau
+  dbl4p gt[10];
+#pragma unroll
+  for (int p = 0; p < 10; p++) gt[p] = (dbl4p){0.0, 0.0, 0.0, 0.0};
+  gram_tiles(vb, gt);
+  gram_sum_store(gt, Gp, gram_part + (size_t)blockIdx.x * KMAX * KMAX);
 }
 
 // The block reflector of the Gram-based panel from the V it stored: S = V^T V
