@@ -36,7 +36,8 @@
 //
 // The one-workgroup stages keep 64 x 64 matrices in LDS and work in
 // 16-column blocks: wave 0 factors a diagonal block with one lane per row,
-// taking the pivot and the pivot row from lane i with readlane (no barrier
+// taking the pivot and the pivot row from lane i by a DPP row broadcast
+// (register to register, all in vector registers; no barrier
 // inside the block; each column's reciprocal or inverse square root is a
 // hardware estimate refined by Newton steps, and the column's results are
 // computed before its stop test, which only decides what is kept); all
@@ -126,40 +127,41 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
     if (wave == 0) {
       // lane r (= lane % 16) holds row bs + r of the symmetric diagonal block;
       // both triangles are updated, so R(i, r) = S(r, i) rs is the lane's own
-      // value and row i comes from lane i by readlane
-      const int r = lane & 15;
+      // value and row i comes from lane i by a row broadcast
+      // (re-read per block: a row index fixed across the blocks lets the
+      // compiler hoist all 32 row masks out of the loop and spill them)
+      const int r = local_tid() & 15;
       double w[BLK];
 #pragma unroll
       for (int c = 0; c < BLK; c++) w[c] = S[bs + r][bs + c];
+      // Branch-free pivot steps, as in the LU below: a pivot that is not
+      // taken updates nothing (zero multipliers; entries past the stop are
+      // not read), and lane r stores 1 / R(r, r) after the block.
       double rsr = 0.0;  // 1 / sqrt(pivot) of this lane's row once eliminated
       bool act = true;
       int done = BLK, code = 0;
 #pragma unroll
       for (int i = 0; i < BLK; i++) {
         const int gi = bs + i;
-        const double d = rdlane(w[i], i);
+        const double d = row_bcast(w[i], i);
         const int stc = (gi < nlim) ? stopfn(gi, d) : 0;
         const double rs = rsqrt_nr(d);  // committed only when the pivot is accepted
         const bool go = act && gi < nlim && stc == 0;
-        if (act && !go) {
-          done = i;
-          code = stc;
-        }
+        done = (act && !go) ? i : done;
+        code = (act && !go) ? stc : code;
         act = go;
-        if (go) {
-          if (lane == 0) rinv[gi] = rs;
-          rsr = (r == i) ? rs : rsr;
-          // S(r, c) -= R(i, r) R(i, c) = S(r, i) rs^2 S(i, c) for rows r > i
-          const double a = (r > i) ? w[i] * rs * rs : 0.0;
+        rsr = (r == i) ? rs : rsr;
+        // S(r, c) -= R(i, r) R(i, c) = S(r, i) rs^2 S(i, c) for rows r > i
+        const double a = (go && r > i) ? w[i] * rs * rs : 0.0;
 #pragma unroll
-          for (int c = i + 1; c < BLK; c++) w[c] = fma(-a, rdlane(w[c], i), w[c]);
-        }
+        for (int c = i + 1; c < BLK; c++) w[c] = fma(-a, row_bcast(w[c], i), w[c]);
       }
       // rows < done: R(r, c) = S(r, c) / sqrt(pivot) on and above the diagonal
       if (lane < BLK) {
 #pragma unroll
         for (int c = 0; c < BLK; c++)
           S[bs + r][bs + c] = (r < done) ? ((c < r) ? 0.0 : w[c] * rsr) : w[c];
+        if (r < done) rinv[bs + r] = rsr;
       }
       if (lane == 0) {
         *s_done = done;
@@ -215,8 +217,10 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
   for (int bs = 0; bs < nlim; bs += BLK) {
     if (wave == 0) {
       // lane r (= lane % 16) holds row bs + r of the diagonal block; the pivot
-      // row comes from lane i by readlane
-      const int r = lane & 15;
+      // row comes from lane i by a row broadcast
+      // (re-read per block: a row index fixed across the blocks lets the
+      // compiler hoist all 32 row masks out of the loop and spill them)
+      const int r = local_tid() & 15;
       double w[BLK];
 #pragma unroll
       for (int c = 0; c < BLK; c++) w[c] = W[bs + r][bs + c];
@@ -227,41 +231,48 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
         const double rt = thres / SR[bs + r][bs + r];
         rt2 = rt * rt;
       }
+      // Branch-free pivot steps: the stop code is a select, and a pivot that
+      // is not taken updates nothing (its multipliers are zero; entries past
+      // the stop are not read).  Lane r keeps d, tau and the scale of its own
+      // column and stores them after the block.
       bool act = true;
       int done = BLK, code = 0;
+      double d_r = 0.0, tl_r = 0.0, sc_r = 0.0;
 #pragma unroll
       for (int i = 0; i < BLK; i++) {
         const int gi = bs + i;
-        const double al = rdlane(w[i], i);
+        const double al = row_bcast(w[i], i);
         const double aa = fabs(al);
         const double om = 1.0 - aa;  // 2 - tau
         const double x2 = fmax(om, 0.0) * (1.0 + aa);  // (||x_i|| / R(i, i))^2
         const double di = (al >= 0.0) ? -1.0 : 1.0;
         const double sc = -di * rcp_nr12(1.0 + aa);  // d_i / (-1 - |a_i|)
-        const int stc = (gi >= nlim) ? 0 : (!(al == al) ? 2 : ((gi >= 1 && x2 < rdlane(rt2, i)) ? 1 : (
-                        // tau near 2: dlarfg's tau = 0 branch or lost digits
-                        (om < 1e-6) ? 2 : 0)));
+        const double rt2i = row_bcast(rt2, i);
+        // non-finite pivot, the truncation test, tau near 2 (dlarfg's tau = 0
+        // branch or lost digits)
+        const int stc = (gi >= nlim) ? 0
+                                     : (!(al == al) ? 2 : ((gi >= 1 && x2 < rt2i) ? 1
+                                                                                 : ((om < 1e-6) ? 2 : 0)));
         const bool go = act && gi < nlim && stc == 0;
-        if (act && !go) {
-          done = i;
-          code = stc;
-        }
+        done = (act && !go) ? i : done;
+        code = (act && !go) ? stc : code;
         act = go;
-        if (go) {
-          if (lane == 0) {
-            dsg[gi] = di;
-            tl[gi] = 1.0 + aa;
-            scv[gi] = sc;
-          }
-          const double lv = (r > i) ? w[i] * sc : 0.0;  // L(r, i)
+        d_r = (r == i) ? di : d_r;
+        tl_r = (r == i) ? 1.0 + aa : tl_r;
+        sc_r = (r == i) ? sc : sc_r;
+        const double lv = (go && r > i) ? w[i] * sc : 0.0;  // L(r, i)
 #pragma unroll
-          for (int c = i + 1; c < BLK; c++) w[c] = fma(-lv, rdlane(w[c], i), w[c]);
-          w[i] = (r > i) ? lv : w[i];
-        }
+        for (int c = i + 1; c < BLK; c++) w[c] = fma(-lv, row_bcast(w[c], i), w[c]);
+        w[i] = (go && r > i) ? lv : w[i];
       }
       if (lane < BLK) {
 #pragma unroll
         for (int c = 0; c < BLK; c++) W[bs + r][bs + c] = w[c];
+        if (r < done) {
+          dsg[bs + r] = d_r;
+          tl[bs + r] = tl_r;
+          scv[bs + r] = sc_r;
+        }
       }
       if (lane == 0) {
         *s_done = done;
@@ -428,10 +439,8 @@ __device__ __forceinline__ int chol1_core(double (*S)[LDP], double (*X)[LDP], do
   const int k1 = chol_blocked(
       S, fjb,
       [&](int i, double d) -> int {
-        if (!scale_ok) return 2;
-        if (i >= 1 && d < thr2 && thr2 > tolp) return 1;  // truncation at <= i is certain
-        if (!(d > tolp)) return 2;
-        return 0;
+        // 1: the truncation at <= i is certain; 2: out of range or unresolved
+        return !scale_ok ? 2 : ((i >= 1 && d < thr2 && thr2 > tolp) ? 1 : (!(d > tolp) ? 2 : 0));
       },
       rinv, s_done, s_why, &why);
   if (why == 2) return -1;

@@ -117,6 +117,28 @@ __device__ __forceinline__ double rdlane(double x, int l) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// Value of x in lane i (0 <= i < 16, a compile-time index once the caller's
+// loop is unrolled) of each 16-lane row, in every lane of that row: the DPP
+// row broadcast (row_newbcast:i), register to register, no scalar registers.
+__device__ __forceinline__ int row_bcast_i32(int x, int i) {
+#define QRDM_RB(I) \
+  case I:          \
+    return __builtin_amdgcn_mov_dpp(x, 0x150 + I, 0xF, 0xF, false);
+  switch (i) {
+    QRDM_RB(0) QRDM_RB(1) QRDM_RB(2) QRDM_RB(3) QRDM_RB(4) QRDM_RB(5) QRDM_RB(6) QRDM_RB(7)
+    QRDM_RB(8) QRDM_RB(9) QRDM_RB(10) QRDM_RB(11) QRDM_RB(12) QRDM_RB(13) QRDM_RB(14)
+    default:
+      return __builtin_amdgcn_mov_dpp(x, 0x150 + 15, 0xF, 0xF, false);
+  }
+#undef QRDM_RB
+}
+__device__ __forceinline__ double row_bcast(double x, int i) {
+  const long long b = __double_as_longlong(x);
+  const int lo = row_bcast_i32((int)b, i);
+  const int hi = row_bcast_i32((int)(b >> 32), i);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // Orders a wave's LDS accesses: its earlier stores are visible to its later
 // loads (one wave's LDS operations complete in order).
 __device__ __forceinline__ void wave_lds_sync() {
