@@ -325,8 +325,14 @@ __device__ __forceinline__ int bt_panel_gram(double* a, int lda, int j, int m, i
   // 3. the reconstruction: SR = E0, SA = E1 (R1^-1), SW = TS (also its
   // scratch Y, and M out), SX = RT
   double* Mf = &TS[0][0];  // M(q, c) at Mf[q + c * KMAX]
-  const int k = recon_core(E0, E1, TS, RT, Yof(L.Ts), L.rinv, L.dsg, L.tl, L.scv, &L.s_done,
-                           &L.s_why, nullptr, a, lda, j, fjb, k1, tau_dm, tau + j, Mf);
+  const int k = recon_core(
+      E0, E1, TS, RT, Yof(L.Ts), L.rinv, L.dsg, L.tl, L.scv, &L.s_done, &L.s_why,
+      [&](double(*S)[LDP]) {  // E1 holds R1^-1: S takes it before E1 is overwritten
+        for (int e = tid; e < KMAX * KMAX; e += NW * 64) S[e & 63][e >> 6] = E1[e & 63][e >> 6];
+        lds_barrier();
+      },
+      [&](double(*S)[LDP]) { load_slab(S, a, lda, j, j, j + k1, k1); }, a, lda, j, fjb, k1, tau_dm,
+      tau + j, Mf);
   if (k < 0) return -1;
   __syncthreads();  // the top block the reconstruction wrote to A is read below
   for (int e = tid; e < KMAX * KMAX; e += NW * 64) E0[e & 63][e >> 6] = Mf[e];  // M, padded

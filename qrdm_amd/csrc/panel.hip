@@ -54,6 +54,10 @@ constexpr int PR_ROWS = 64;  // rows per workgroup of passes 2 and 3
 constexpr double kEpsP = 1.1102230246251565e-16;
 constexpr int LDP = KMAX + 1;  // padded leading dimension of LDS matrices
 constexpr int BLK = 16;        // block width of the one-workgroup factorisations
+// Threads of every workgroup that runs this file's one-workgroup stages (the
+// launches here and the batched kernel's four waves): element loops over a
+// 64 x 64 matrix stride by it, so that they unroll and their loads overlap.
+constexpr int NT1 = 256;
 
 typedef double dbl4p __attribute__((ext_vector_type(4)));
 
@@ -177,7 +181,7 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
     const int lo = bs + BLK;
     if (lo >= nlim) break;
     // block row: R(bs.., c) = R_bb^-T S(bs.., c), one thread per column
-    for (int c = lo + tid; c < nlim; c += blockDim.x) {
+    for (int c = lo + tid; c < nlim; c += NT1) {
       double x[BLK];
 #pragma unroll
       for (int i = 0; i < BLK; i++) x[i] = S[bs + i][c];
@@ -355,7 +359,7 @@ __device__ __forceinline__ void lu_inverse(const double (*W)[LDP], double (*X)[L
   };
   if (tid < KMAX) rinv[tid] = 1.0 / W[tid][tid];
   if (!BOTH) {
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e & 63, c = e >> 6;
       if (r > c) X[r][c] = 0.0;
     }
@@ -445,7 +449,7 @@ __device__ __forceinline__ int chol1_core(double (*S)[LDP], double (*X)[LDP], do
       rinv, s_done, s_why, &why);
   if (why == 2) return -1;
   // R1 (zero below the diagonal), identity-padded beyond k1 for the inverse
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+  for (int e = tid; e < KMAX * KMAX; e += NT1) {
     const int r = e >> 6, c = e & 63;
     const bool in = r < k1 && c < k1;
     const double v = (in && c >= r) ? S[r][c] : 0.0;
@@ -470,25 +474,28 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
   if (!step_runs(st, RUN_FAST)) return;
   const int fjb = st->fjb;
   const int tid = threadIdx.x;
-  if (tid < KMAX) {
-    const int rk = tid < fjb ? rank[tid] : 0;
-    srk[tid] = rk;
-    diag[tid] = tid < fjb ? Gc[rk * (KMAX + 1)] : 0.0;
+  // the whole candidate Gram and the rank map in one round trip (X holds the
+  // Gram until the inverse overwrites it), then the gather from LDS
+  {
+    const int rk = (tid < fjb) ? rank[tid] : 0;
+    double gv[KMAX * KMAX / NT1];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) gv[q] = Gc[tid + NT1 * q];
+    if (tid < KMAX) srk[tid] = rk;
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e = tid + NT1 * q;
+      X[e & 63][e >> 6] = gv[q];  // X[r][c] = Gc(r, c)
+    }
   }
   lds_barrier();
-  {
-    double gv[KMAX * KMAX / 256];
+  if (tid < KMAX) diag[tid] = tid < fjb ? X[srk[tid]][srk[tid]] : 0.0;
 #pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
-      gv[q] = (r < fjb && c < fjb) ? Gc[srk[r] + srk[c] * KMAX] : (r == c ? 1.0 : 0.0);
-    }
-#pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      const int e = tid + 256 * q;
-      S[e >> 6][e & 63] = gv[q];
-    }
+  for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+    const int e = tid + NT1 * q, r = e >> 6, c = e & 63;
+    S[r][c] = (r < fjb && c < fjb) ? X[srk[r]][srk[c]] : (r == c ? 1.0 : 0.0);
   }
+  lds_barrier();
   const int k1 = chol1_core(S, X, Y, rinv, diag, fjb, tau_dm, &s_done, &s_why);
   if (k1 < 0) {
     if (tid == 0) {
@@ -497,7 +504,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
     }
     return;
   }
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+  for (int e = tid; e < KMAX * KMAX; e += NT1) {
     const int r = e >> 6, c = e & 63;
     const bool in = r < k1 && c < k1;
     R1[r + c * KMAX] = (in && c >= r) ? S[r][c] : 0.0;
@@ -710,18 +717,21 @@ __global__ __launch_bounds__(256) void k_qgram(const double* __restrict__ A, int
 
 // ---------------------------------------------------------------------------
 // Reconstruction (see the file header) on LDS matrices: SX = G2 (identity
-// beyond k1), SR = R1 (zero beyond k1); R1inv: R1^-1 in memory (column-major,
-// leading dimension KMAX, zero outside k1 x k1), or nullptr when SA already
-// holds it.  Writes the top k x k block of the panel at column j (R above and
-// on the diagonal, V1 below), tau[0..k) and M (column-major, leading
-// dimension KMAX) to M_out, and returns k; or returns -1, with nothing
-// written, when the Gram-based form falls back.  Y may share SW's storage.
-// The caller's LDS writes are ordered by a barrier before the call.
+// beyond k1), SR = R1 (zero beyond k1).  r1inv_into(SX) writes R1^-1 (upper,
+// zero outside k1 x k1) into SX, reading SA where SA holds it (it then ends
+// with a barrier; SA is overwritten after the call); top_into(SW) writes the
+// panel's top block A(j + r, j + c), r, c < k1, zero elsewhere.  Writes the
+// top k x k block of the panel at column j (R above and on the diagonal, V1
+// below), tau[0..k) and M (column-major, leading dimension KMAX) to M_out,
+// and returns k; or returns -1, with nothing written, when the Gram-based
+// form falls back.  Y may share SW's storage.  The caller's LDS writes are
+// ordered by a barrier before the call.
+template <class R1InvFn, class TopFn>
 __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], double (*SW)[LDP],
                           double (*SX)[LDP], double (*Y)[BLK][BLK + 1], double* rinv,
                           double* dsg, double* tl, double* scv, int* s_done, int* s_why,
-                          const double* R1inv, double* A, int lda, int j, int fjb, int k1,
-                          double tau_dm, double* tau, double* M_out) {
+                          R1InvFn r1inv_into, TopFn top_into, double* A, int lda, int j,
+                          int fjb, int k1, double tau_dm, double* tau, double* M_out) {
   const int tid = local_tid();
   // 1-3. R = R2 R1 with R2^T R2 = G2 and R^-1 (into SR and SA).  After the
   // first pass G2 = I + E with E tiny: then R2 = I + F, F upper triangular
@@ -733,7 +743,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
   double emax;
   {
     double em = 0.0;
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e >> 6, c = e & 63;
       em = fmax(em, fabs(SX[r][c] - (r == c ? 1.0 : 0.0)));
     }
@@ -755,7 +765,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
     int npass = 0;
     for (double err = emax * emax; err > 8.7e-19 && npass < 6; err *= emax) npass++;
     // SW = E, SX = F = triu_half(E)
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e >> 6, c = e & 63;
       const double ev = SX[r][c] - (r == c ? 1.0 : 0.0);
       SW[r][c] = ev;
@@ -792,7 +802,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
         for (int v = 0; v < 4; v++) if (t < nj) SR[BLK * jb[t].ti + g + 4 * v][BLK * jb[t].tj + li] += acc[t][v];
     }
     // R2^-1 = I - F (I - F (...)) into SW (upper)
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e >> 6, c = e & 63;
       SW[r][c] = (r == c ? 1.0 : 0.0) - SX[r][c];
     }
@@ -812,13 +822,8 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
       lds_barrier();
     }
     // R^-1 = R1^-1 R2^-1 (both upper) into SA, zero below the diagonal blocks
-    if (R1inv) {
-      load_mat(SX, R1inv);
-    } else {  // SA holds R1^-1: SX takes it before SA is overwritten
-      for (int e = tid; e < KMAX * KMAX; e += blockDim.x) SX[e & 63][e >> 6] = SA[e & 63][e >> 6];
-      lds_barrier();
-    }
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    r1inv_into(SX);
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e & 63, c = e >> 6;
       if ((r >> 4) > (c >> 4)) SA[r][c] = 0.0;
     }
@@ -838,7 +843,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
         SX, k1, [&](int, double d) -> int { return (d > 0.25 && d < 4.0) ? 0 : 2; }, rinv,
         s_done, s_why, &why);
     if (why) return -1;
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e >> 6, c = e & 63;
       if (!(r < k1 && c < k1 && c >= r)) SX[r][c] = 0.0;
     }
@@ -852,7 +857,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
     lu_inverse<false>(SR, SA, rinv, Y);  // SA = R^-1 (upper part)
   }
   // 4. Q11 = P_top R^-1 -> SW
-  load_slab(SW, A, lda, j, j, j + k1, k1);
+  top_into(SW);
   lds_barrier();
   {
     TileJob jq[4];
@@ -873,7 +878,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
   if (why == 2) return -1;
   // 6. top block (R rows scaled by the signs, V1 below), tau; SX := the
   // signed U (on and above the diagonal) and L (below), identity beyond k
-  for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+  for (int e = tid; e < KMAX * KMAX; e += NT1) {
     const int r = e & 63, c = e >> 6;  // consecutive threads: consecutive rows
     const bool in = r < k && c < k;
     const double w = SW[r][c];
@@ -916,7 +921,7 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
         }
       }
     }
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = e & 63, c = e >> 6;
       if ((r >> 4) > (c >> 4)) M_out[r + c * KMAX] = 0.0;
     }
@@ -940,26 +945,40 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   const int j = st->j, fjb = st->fjb;
   const int tid = threadIdx.x;
   const int k1 = st->k1;
+  // every global read of the reconstruction in one round trip: G2, R1, and,
+  // kept in registers until their stage, R1^-1 and the panel's top block
+  // (e walks rows fastest: coalesced column-major reads)
+  double pri[KMAX * KMAX / NT1], ptop[KMAX * KMAX / NT1];
   {
-    double gx[KMAX * KMAX / 256], gr[KMAX * KMAX / 256];
+    double gx[KMAX * KMAX / NT1], gr[KMAX * KMAX / NT1];
 #pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      // e walks rows fastest: coalesced column-major reads
-      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e = tid + NT1 * q, r = e & 63, c = e >> 6;
       const bool in = r < k1 && c < k1;
       gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);  // from the launch's reduce
       gr[q] = R1[r + c * KMAX];
+      pri[q] = R1inv[r + c * KMAX];
+      ptop[q] = in ? A[(size_t)(j + c) * lda + j + r] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < KMAX * KMAX / 256; q++) {
-      const int e = tid + 256 * q, r = e & 63, c = e >> 6;
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e = tid + NT1 * q, r = e & 63, c = e >> 6;
       SX[r][c] = gx[q];
       SR[r][c] = gr[q];
     }
   }
   lds_barrier();
-  const int k = recon_core(SR, SA, SW, SX, Y, rinv, dsg, tl, scv, &s_done, &s_why, R1inv, A, lda,
-                           j, fjb, k1, tau_dm, tau + j, M_out);
+  auto from_regs = [&](const double(&v)[KMAX * KMAX / NT1], double(*S)[LDP]) {
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e = tid + NT1 * q;
+      S[e & 63][e >> 6] = v[q];
+    }
+  };
+  const int k = recon_core(
+      SR, SA, SW, SX, Y, rinv, dsg, tl, scv, &s_done, &s_why,
+      [&](double(*S)[LDP]) { from_regs(pri, S); }, [&](double(*S)[LDP]) { from_regs(ptop, S); }, A,
+      lda, j, fjb, k1, tau_dm, tau + j, M_out);
   if (k < 0) {
     if (tid == 0) {
       st->flags |= 2;  // the exact panel takes the step over
@@ -1038,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_v2(double* A, int lda, int m,
     // V is zero above the panel: the trailing update's slabs start at the
     // 32-row boundary at or below j and rely on it
     const int z0 = j >= KMAX ? j - KMAX : 0;
-    for (int e = tid; e < KMAX * KMAX; e += blockDim.x) {
+    for (int e = tid; e < KMAX * KMAX; e += NT1) {
       const int r = z0 + (e & 63), c = e >> 6;
       if (r < j) V[(size_t)c * ldv + r] = 0.0;
     }
@@ -1129,21 +1148,25 @@ __global__ __launch_bounds__(256) void k_reduce_t(const double* __restrict__ par
   if (!s_last) return;
   const int k = st->fjb_cmp;
   const int tid = threadIdx.x;
-  for (int e2 = tid; e2 < KMAX * KMAX; e2 += blockDim.x) {
-    const int r = e2 & 63, c = e2 >> 6;  // consecutive threads: consecutive rows
-    double x;
-    if (r < k && c < k) {
-      const double sv = ld_dev(Sg + r + c * KMAX);
-      x = (r < c) ? sv : (r == c ? 0.5 * sv : 0.0);
-    } else {
-      x = (r == c) ? 1.0 : 0.0;
+  {
+    // every load in flight before the LDS stores (one round trip)
+    double sv[KMAX * KMAX / NT1];
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e2 = tid + NT1 * q, r = e2 & 63, c = e2 >> 6;  // consecutive rows
+      sv[q] = (r < k && c < k) ? ld_dev(Sg + r + c * KMAX) : 0.0;
     }
-    SX[r][c] = x;
+#pragma unroll
+    for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+      const int e2 = tid + NT1 * q, r = e2 & 63, c = e2 >> 6;
+      SX[r][c] = (r < k && c < k) ? ((r < c) ? sv[q] : (r == c ? 0.5 * sv[q] : 0.0))
+                                  : ((r == c) ? 1.0 : 0.0);
+    }
   }
   lds_barrier();
   if (tid < k) tau[j + tid] = 1.0 / SX[tid][tid];  // 2 / S_ii
   lu_inverse<false>(SX, SY, rinv, Y);  // SY = the upper triangular inverse
-  for (int e2 = tid; e2 < KMAX * KMAX; e2 += blockDim.x) {
+  for (int e2 = tid; e2 < KMAX * KMAX; e2 += NT1) {
     const int r = e2 & 63, c = e2 >> 6;
     T_out[r + c * KMAX] = (r < k && c < k && r <= c) ? SY[r][c] : 0.0;
   }
