@@ -89,21 +89,31 @@ __device__ __forceinline__ double rcp_nr12(double x) {
 }
 
 // S[r][c] -= sum_{t < 16} A(r, t) B(t, c) for r, c in [lo, nlim), 16 x 16
-// tiles on the matrix cores (wave w takes tiles w, w + 4, ...).
-// sym: A(r, t) = S[bs + t][r] and only tiles on or above the diagonal (the
-// symmetric Cholesky update); otherwise A(r, t) = S[r][bs + t].
-// B(t, c) = S[bs + t][c].  The tiles written are disjoint from the rows
-// bs..bs+15 and columns bs..bs+15 read.
+// tiles on the matrix cores.  sym: A(r, t) = S[bs + t][r] and only tiles on
+// or above the diagonal (the symmetric Cholesky update); otherwise A(r, t) =
+// S[r][bs + t].  B(t, c) = S[bs + t][c].  The tiles written are disjoint
+// from the rows bs..bs+15 and columns bs..bs+15 read.  The blocked
+// factorisations look one block ahead: the calling wave takes only the next
+// diagonal tile (lo, lo) with diag_tile, and waves 1-3 share the other
+// tiles (wave w takes the tiles w - 1, w + 2, ... of them) without it.
 __device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, int nlim,
-                                              bool sym) {
+                                              bool sym, bool diag_tile) {
   const int lane = local_tid() & 63, wave = local_tid() >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int nt = (KMAX - lo) >> 4;
   int e = 0;
   for (int tr = 0; tr < nt; tr++) {
-    for (int tc = sym ? tr : 0; tc < nt; tc++, e++) {
+    for (int tc = sym ? tr : 0; tc < nt; tc++) {
       const int r0 = lo + 16 * tr, c0 = lo + 16 * tc;
-      if ((e & 3) != wave || r0 >= nlim || c0 >= nlim) continue;
+      const bool first = tr == 0 && tc == 0;
+      bool mine;
+      if (diag_tile) {
+        mine = first;
+      } else {
+        mine = !first && (e % 3) == wave - 1;
+        if (!first) e++;
+      }
+      if (!mine || r0 >= nlim || c0 >= nlim) continue;
       dbl4p acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int k0 = 0; k0 < BLK; k0 += 4) {
@@ -126,51 +136,62 @@ __device__ __forceinline__ void mfma_trailing(double (*S)[LDP], int bs, int lo, 
 template <class StopFn>
 __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn stopfn, double* rinv,
                                             int* s_done, int* s_why, int* why) {
-  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), wave = tid >> 6;
+  // Wave 0 factors the diagonal block at bs: lane r (= lane % 16) holds row
+  // bs + r of the symmetric block; both triangles are updated, so R(i, r) =
+  // S(r, i) rs is the lane's own value and row i comes from lane i by a row
+  // broadcast.  Branch-free pivot steps: a pivot that is not taken updates
+  // nothing (zero multipliers; entries past the stop are not read), and lane
+  // r stores 1 / R(r, r) after the block.  (The row index is re-read per
+  // block: one fixed across the blocks lets the compiler hoist all 32 row
+  // masks out of the loop and spill them.)
+  auto diag = [&](int bs) {
+    const int lane = local_tid() & 63, r = lane & 15;
+    double w[BLK];
+#pragma unroll
+    for (int c = 0; c < BLK; c++) w[c] = S[bs + r][bs + c];
+    double rsr = 0.0;  // 1 / sqrt(pivot) of this lane's row once eliminated
+    bool act = true;
+    int done = BLK, code = 0;
+#pragma unroll
+    for (int i = 0; i < BLK; i++) {
+      const int gi = bs + i;
+      const double d = row_bcast(w[i], i);
+      const int stc = (gi < nlim) ? stopfn(gi, d) : 0;
+      const double rs = rsqrt_nr(d);  // committed only when the pivot is accepted
+      const bool go = act && gi < nlim && stc == 0;
+      done = (act && !go) ? i : done;
+      code = (act && !go) ? stc : code;
+      act = go;
+      rsr = (r == i) ? rs : rsr;
+      // S(r, c) -= R(i, r) R(i, c) = S(r, i) rs^2 S(i, c) for rows r > i
+      const double a = (go && r > i) ? w[i] * rs * rs : 0.0;
+#pragma unroll
+      for (int c = i + 1; c < BLK; c++) w[c] = fma(-a, row_bcast(w[c], i), w[c]);
+    }
+    // rows < done: R(r, c) = S(r, c) / sqrt(pivot) on and above the diagonal
+    if (lane < BLK) {
+#pragma unroll
+      for (int c = 0; c < BLK; c++)
+        S[bs + r][bs + c] = (r < done) ? ((c < r) ? 0.0 : w[c] * rsr) : w[c];
+      if (r < done) rinv[bs + r] = rsr;
+    }
+    if (lane == 0) {
+      *s_done = done;
+      *s_why = code;
+    }
+  };
   for (int bs = 0; bs < nlim; bs += BLK) {
+    // look-ahead: the previous block's trailing update, wave 0 on this
+    // diagonal tile (then its factorisation), waves 1-3 on the other tiles
     if (wave == 0) {
-      // lane r (= lane % 16) holds row bs + r of the symmetric diagonal block;
-      // both triangles are updated, so R(i, r) = S(r, i) rs is the lane's own
-      // value and row i comes from lane i by a row broadcast
-      // (re-read per block: a row index fixed across the blocks lets the
-      // compiler hoist all 32 row masks out of the loop and spill them)
-      const int r = local_tid() & 15;
-      double w[BLK];
-#pragma unroll
-      for (int c = 0; c < BLK; c++) w[c] = S[bs + r][bs + c];
-      // Branch-free pivot steps, as in the LU below: a pivot that is not
-      // taken updates nothing (zero multipliers; entries past the stop are
-      // not read), and lane r stores 1 / R(r, r) after the block.
-      double rsr = 0.0;  // 1 / sqrt(pivot) of this lane's row once eliminated
-      bool act = true;
-      int done = BLK, code = 0;
-#pragma unroll
-      for (int i = 0; i < BLK; i++) {
-        const int gi = bs + i;
-        const double d = row_bcast(w[i], i);
-        const int stc = (gi < nlim) ? stopfn(gi, d) : 0;
-        const double rs = rsqrt_nr(d);  // committed only when the pivot is accepted
-        const bool go = act && gi < nlim && stc == 0;
-        done = (act && !go) ? i : done;
-        code = (act && !go) ? stc : code;
-        act = go;
-        rsr = (r == i) ? rs : rsr;
-        // S(r, c) -= R(i, r) R(i, c) = S(r, i) rs^2 S(i, c) for rows r > i
-        const double a = (go && r > i) ? w[i] * rs * rs : 0.0;
-#pragma unroll
-        for (int c = i + 1; c < BLK; c++) w[c] = fma(-a, row_bcast(w[c], i), w[c]);
+      if (bs > 0) {
+        mfma_trailing(S, bs - BLK, bs, nlim, true, true);
+        wave_lds_sync();
       }
-      // rows < done: R(r, c) = S(r, c) / sqrt(pivot) on and above the diagonal
-      if (lane < BLK) {
-#pragma unroll
-        for (int c = 0; c < BLK; c++)
-          S[bs + r][bs + c] = (r < done) ? ((c < r) ? 0.0 : w[c] * rsr) : w[c];
-        if (r < done) rinv[bs + r] = rsr;
-      }
-      if (lane == 0) {
-        *s_done = done;
-        *s_why = code;
-      }
+      diag(bs);
+    } else if (bs > 0) {
+      mfma_trailing(S, bs - BLK, bs, nlim, true, false);
     }
     lds_barrier();
     const int done = *s_done, code = *s_why;
@@ -196,8 +217,6 @@ __device__ __forceinline__ int chol_blocked(double (*S)[LDP], int nlim, StopFn s
       for (int i = 0; i < BLK; i++) S[bs + i][c] = x[i];
     }
     lds_barrier();
-    mfma_trailing(S, bs, lo, nlim, true);
-    lds_barrier();
   }
   *why = 0;
   return nlim;
@@ -217,71 +236,82 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
                                                  double thres, double* dsg, double* tl,
                                                  double* scv, int* s_done, int* s_why,
                                                  int* why) {
-  const int tid = local_tid(), lane = tid & 63, wave = tid >> 6;
+  const int tid = local_tid(), wave = tid >> 6;
+  // Wave 0 factors the diagonal block at bs: lane r (= lane % 16) holds row
+  // bs + r; the pivot row comes from lane i by a row broadcast.  Branch-free
+  // pivot steps: the stop code is a select, and a pivot that is not taken
+  // updates nothing (its multipliers are zero; entries past the stop are not
+  // read).  Lane r keeps d, tau and the scale of its own column and stores
+  // them after the block.  (The row index is re-read per block: one fixed
+  // across the blocks lets the compiler hoist all 32 row masks out of the
+  // loop and spill them.)
+  auto diag = [&](int bs) {
+    const int lane = local_tid() & 63, r = lane & 15;
+    double w[BLK];
+#pragma unroll
+    for (int c = 0; c < BLK; c++) w[c] = W[bs + r][bs + c];
+    // ||x_i|| < thres  <=>  (1 - a^2) < (thres / R(i, i))^2: lane r holds the
+    // bound of column bs + r (R's diagonal is positive)
+    double rt2;
+    {
+      const double rt = thres / SR[bs + r][bs + r];
+      rt2 = rt * rt;
+    }
+    bool act = true;
+    int done = BLK, code = 0;
+    double d_r = 0.0, tl_r = 0.0, sc_r = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLK; i++) {
+      const int gi = bs + i;
+      const double al = row_bcast(w[i], i);
+      const double aa = fabs(al);
+      const double om = 1.0 - aa;  // 2 - tau
+      const double x2 = fmax(om, 0.0) * (1.0 + aa);  // (||x_i|| / R(i, i))^2
+      const double di = (al >= 0.0) ? -1.0 : 1.0;
+      const double sc = -di * rcp_nr12(1.0 + aa);  // d_i / (-1 - |a_i|)
+      const double rt2i = row_bcast(rt2, i);
+      // non-finite pivot, the truncation test, tau near 2 (dlarfg's tau = 0
+      // branch or lost digits)
+      const int stc = (gi >= nlim) ? 0
+                                   : (!(al == al) ? 2 : ((gi >= 1 && x2 < rt2i) ? 1
+                                                                               : ((om < 1e-6) ? 2 : 0)));
+      const bool go = act && gi < nlim && stc == 0;
+      done = (act && !go) ? i : done;
+      code = (act && !go) ? stc : code;
+      act = go;
+      d_r = (r == i) ? di : d_r;
+      tl_r = (r == i) ? 1.0 + aa : tl_r;
+      sc_r = (r == i) ? sc : sc_r;
+      const double lv = (go && r > i) ? w[i] * sc : 0.0;  // L(r, i)
+#pragma unroll
+      for (int c = i + 1; c < BLK; c++) w[c] = fma(-lv, row_bcast(w[c], i), w[c]);
+      w[i] = (go && r > i) ? lv : w[i];
+    }
+    if (lane < BLK) {
+#pragma unroll
+      for (int c = 0; c < BLK; c++) W[bs + r][bs + c] = w[c];
+      if (r < done) {
+        dsg[bs + r] = d_r;
+        tl[bs + r] = tl_r;
+        scv[bs + r] = sc_r;
+      }
+    }
+    if (lane == 0) {
+      *s_done = done;
+      *s_why = code;
+    }
+  };
   for (int bs = 0; bs < nlim; bs += BLK) {
+    // look-ahead: the previous block's trailing update, wave 0 on this
+    // diagonal tile (then its factorisation), waves 1-3 on the other tiles
     if (wave == 0) {
-      // lane r (= lane % 16) holds row bs + r of the diagonal block; the pivot
-      // row comes from lane i by a row broadcast
-      // (re-read per block: a row index fixed across the blocks lets the
-      // compiler hoist all 32 row masks out of the loop and spill them)
-      const int r = local_tid() & 15;
-      double w[BLK];
-#pragma unroll
-      for (int c = 0; c < BLK; c++) w[c] = W[bs + r][bs + c];
-      // ||x_i|| < thres  <=>  (1 - a^2) < (thres / R(i, i))^2: lane r holds the
-      // bound of column bs + r (R's diagonal is positive)
-      double rt2;
-      {
-        const double rt = thres / SR[bs + r][bs + r];
-        rt2 = rt * rt;
+      if (bs > 0) {
+        mfma_trailing(W, bs - BLK, bs, nlim, false, true);
+        wave_lds_sync();
       }
-      // Branch-free pivot steps: the stop code is a select, and a pivot that
-      // is not taken updates nothing (its multipliers are zero; entries past
-      // the stop are not read).  Lane r keeps d, tau and the scale of its own
-      // column and stores them after the block.
-      bool act = true;
-      int done = BLK, code = 0;
-      double d_r = 0.0, tl_r = 0.0, sc_r = 0.0;
-#pragma unroll
-      for (int i = 0; i < BLK; i++) {
-        const int gi = bs + i;
-        const double al = row_bcast(w[i], i);
-        const double aa = fabs(al);
-        const double om = 1.0 - aa;  // 2 - tau
-        const double x2 = fmax(om, 0.0) * (1.0 + aa);  // (||x_i|| / R(i, i))^2
-        const double di = (al >= 0.0) ? -1.0 : 1.0;
-        const double sc = -di * rcp_nr12(1.0 + aa);  // d_i / (-1 - |a_i|)
-        const double rt2i = row_bcast(rt2, i);
-        // non-finite pivot, the truncation test, tau near 2 (dlarfg's tau = 0
-        // branch or lost digits)
-        const int stc = (gi >= nlim) ? 0
-                                     : (!(al == al) ? 2 : ((gi >= 1 && x2 < rt2i) ? 1
-                                                                                 : ((om < 1e-6) ? 2 : 0)));
-        const bool go = act && gi < nlim && stc == 0;
-        done = (act && !go) ? i : done;
-        code = (act && !go) ? stc : code;
-        act = go;
-        d_r = (r == i) ? di : d_r;
-        tl_r = (r == i) ? 1.0 + aa : tl_r;
-        sc_r = (r == i) ? sc : sc_r;
-        const double lv = (go && r > i) ? w[i] * sc : 0.0;  // L(r, i)
-#pragma unroll
-        for (int c = i + 1; c < BLK; c++) w[c] = fma(-lv, row_bcast(w[c], i), w[c]);
-        w[i] = (go && r > i) ? lv : w[i];
-      }
-      if (lane < BLK) {
-#pragma unroll
-        for (int c = 0; c < BLK; c++) W[bs + r][bs + c] = w[c];
-        if (r < done) {
-          dsg[bs + r] = d_r;
-          tl[bs + r] = tl_r;
-          scv[bs + r] = sc_r;
-        }
-      }
-      if (lane == 0) {
-        *s_done = done;
-        *s_why = code;
-      }
+      diag(bs);
+    } else if (bs > 0) {
+      mfma_trailing(W, bs - BLK, bs, nlim, false, false);
     }
     lds_barrier();
     const int done = *s_done, code = *s_why;
@@ -326,8 +356,6 @@ __device__ __forceinline__ int lu_signed_blocked(double (*W)[LDP], int nlim, con
         for (int i = 0; i < BLK; i++) W[bs + i][c] = y[i];
       }
     }
-    lds_barrier();
-    mfma_trailing(W, bs, lo, nlim, false);
     lds_barrier();
   }
   *why = 0;
