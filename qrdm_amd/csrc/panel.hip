@@ -459,8 +459,10 @@ __device__ __forceinline__ int chol1_core(double (*S)[LDP], double (*X)[LDP], do
                           double* rinv, const double* diag, int fjb, double tau_dm, int* s_done,
                           int* s_why) {
   const int tid = local_tid();
-  double maxd = 0.0;
-  for (int q = 0; q < KMAX; q++) maxd = fmax(maxd, diag[q]);
+  // the largest diagonal entry (>= 0), a max over each wave's 64 lanes
+  double maxd = fmax(diag[tid & 63], 0.0);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) maxd = fmax(maxd, __shfl_xor(maxd, o, 64));
   const double g00 = diag[0];
   const double tolp = 1e4 * 64.0 * kEpsP * fjb * maxd;  // unresolved pivots
   const double thres = (tau_dm > 0.0 && fjb > 1) ? tau_dm * sqrt(g00) : 5e-14;
@@ -533,7 +535,7 @@ __device__ void chol1_body(const double* __restrict__ Gc, const int* __restrict_
     return;
   }
   for (int e = tid; e < KMAX * KMAX; e += NT1) {
-    const int r = e >> 6, c = e & 63;
+    const int r = e & 63, c = e >> 6;  // consecutive lanes: consecutive rows (coalesced)
     const bool in = r < k1 && c < k1;
     R1[r + c * KMAX] = (in && c >= r) ? S[r][c] : 0.0;
     R1inv[r + c * KMAX] = in ? X[r][c] : 0.0;
