@@ -507,7 +507,7 @@ __device__ __forceinline__ int bt_factor(const BatchItem& it, double* A, double*
                      P.trace + (size_t)slot * POL_REC);
     }
     // --- greedy acceptance and the permute_marked plan
-    if (wave == 0) select_wave(L.R1, P.delta, m, n, INT_MAX, 1, L.perm, &L.st);
+    if (wave == 0) select_wave(L.R1, select_inputs(&L.st, P.delta), m, n, INT_MAX, 1, L.perm, &L.st);
     __syncthreads();
     const int fjb = L.st.fjb, npair = L.st.npair;
     // --- column moves (whole columns; work and jpvt with them, not workold)

@@ -622,25 +622,45 @@ void launch_gram(const double* base, int ld, int r0, int r1, const int* cols, in
 // 3. the plan: (dst, src) pairs of the moved columns into perm[0..4*KMAX),
 //    the candidate rank of each panel column into perm[4*KMAX..5*KMAX), and
 //    the panel form (Gram-based when fjb >= 2 and m - j >= fast_rows).
+// What select_wave reads of the step's state, lane t of the wave: read
+// where the caller chooses (the reduce's workgroups read it before their
+// ticket, so that it arrives with the partials).
+struct SelIn {
+  int j, nc, kmax, cidx;
+  double delta, cval;
+};
+__device__ __forceinline__ SelIn select_inputs(const DevState* st, double delta) {
+  const int t = local_tid() & 63;
+  SelIn in;
+  in.j = st->j;
+  in.nc = st->nc;
+  in.kmax = st->kmax;
+  in.delta = st->policy_on ? st->delta_step : delta;
+  in.cval = st->cand_val[t];
+  in.cidx = st->cand_idx[t];
+  return in;
+}
+
 // Run by one wave (threadIdx.x < 64); its LDS steps are ordered by
 // wave_lds_sync, so the rest of the workgroup may be elsewhere.  Gs: the
 // candidate Gram in LDS, column-major with leading dimension KMAX + 1.
-__device__ __forceinline__ void select_wave(const double* Gs, double delta, int m, int n, int fast_rows,
-                            int exact_queued, int* perm, DevState* st) {
+__device__ __forceinline__ void select_wave(const double* Gs, const SelIn& in, int m, int n,
+                                            int fast_rows, int exact_queued, int* perm,
+                                            DevState* st) {
   __shared__ double s_r[KMAX];
   __shared__ int s_sel[KMAX], s_rank[KMAX];
   __shared__ int s_occ[KMAX], s_slot[KMAX];
   const int t = local_tid();
-  const int j = st->j, nc = st->nc, kmax = st->kmax;
+  const int j = in.j, nc = in.nc, kmax = in.kmax;
   const int nsub = n - j;
-  if (st->policy_on) delta = st->delta_step;
+  const double delta = in.delta;
   // --- 1. greedy.  Lane t builds the mask of earlier candidates s whose
   // |cos(s, t)| >= delta; candidate i is accepted when its mask misses every
   // accepted candidate (max |cos| < delta) and fewer than k_max are accepted.
   unsigned long long acc = 1ull;  // bit i: candidate i accepted
   int nsel = 1;
   if (nc > 1) {
-    const double rt = (t < nc) ? 1.0 / st->cand_val[t] : 0.0;
+    const double rt = (t < nc) ? 1.0 / in.cval : 0.0;
     s_r[t] = rt;
     wave_lds_sync();
     unsigned long long conf = 0ull;
@@ -662,7 +682,7 @@ __device__ __forceinline__ void select_wave(const double* Gs, double delta, int 
   const int fjb = nsel;
   const bool mine = (acc >> t) & 1ull;
   const int spos = __popcll(acc & ((1ull << t) - 1ull));  // acceptance order
-  const int cidx = st->cand_idx[t];
+  const int cidx = in.cidx;
   if (mine) {
     s_sel[spos] = cidx;
     s_rank[spos] = t;
@@ -854,6 +874,14 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
   __shared__ double Gs[KMAX * (KMAX + 1)];
   __shared__ int s_last;
   if (dm_stopped(st)) return;
+  // the step's state for the selection and the W columns to clear, in
+  // flight with the partials
+  SelIn sin;
+  if (threadIdx.x < 64) sin = select_inputs(st, delta);
+  const bool pend = update_pending(st);
+  double* Wq = w_prev(bufs, st);
+  const int jw = st->j, ncw = max(st->nc, 1);
+  const int cid_lane = st->cand_idx[threadIdx.x & 63];
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
   const int pr = e & 63, pc = e >> 6;
   const int es = (pr > pc) ? pc + pr * KMAX : e;  // symmetric: the upper triangle's value
@@ -891,13 +919,15 @@ __global__ __launch_bounds__(256) void k_reduce_select(const double* __restrict_
     }
   }
   __syncthreads();
-  if (threadIdx.x < 64) select_wave(Gs, delta, m, n, fast_rows, exact_queued, perm, st);
-  if (update_pending(st)) {
-    // the candidates received the deferred update in k_gram_cand
-    double* Wq = w_prev(bufs, st);
-    const int j = st->j, nc = max(st->nc, 1);
-    for (int e = threadIdx.x; e < KMAX * nc; e += blockDim.x)
-      Wq[(size_t)(j + st->cand_idx[e >> 6]) * KMAX + (e & 63)] = 0.0;
+  if (threadIdx.x < 64) select_wave(Gs, sin, m, n, fast_rows, exact_queued, perm, st);
+  if (pend) {
+    // the candidates received the deferred update in k_gram_cand: wave w
+    // clears the W columns of candidates w, w + 4, ... (64 entries each)
+    const int lane = threadIdx.x & 63;
+    for (int cq = threadIdx.x >> 6; cq < ncw; cq += 4) {
+      const int ci = __builtin_amdgcn_readlane(cid_lane, cq);
+      Wq[(size_t)(jw + ci) * KMAX + lane] = 0.0;
+    }
   }
 }
 

@@ -959,11 +959,40 @@ __device__ __forceinline__ int recon_core(double (*SR)[LDP], double (*SA)[LDP], 
   return k;
 }
 
-// The reconstruction of the DM loop's Gram-based panel: G2, R1 and R1^-1 from
-// memory, st->fjb_cmp and st->c0, or the fallback flag.
-__device__ void recon_body(const double* __restrict__ G2, const double* __restrict__ R1,
-                           const double* __restrict__ R1inv, double* A, int lda, double tau_dm,
-                           double* tau, DevState* st, double* M_out) {
+// What the reconstruction reads besides G2, all written by earlier kernels
+// of the step: read by every workgroup of the reduce before its ticket, so
+// that it arrives with the partials (only the last workgroup uses it).
+// Column-major 64 x 64 in registers, thread tid holding e = tid + 256 q.
+struct ReconIn {
+  int j, fjb, k1;
+  double gr[KMAX * KMAX / NT1];    // R1
+  double pri[KMAX * KMAX / NT1];   // R1^-1
+  double ptop[KMAX * KMAX / NT1];  // the panel's top block
+};
+__device__ __forceinline__ void recon_inputs(ReconIn& in, const DevState* st,
+                                             const double* __restrict__ R1,
+                                             const double* __restrict__ R1inv,
+                                             const double* A, int lda) {
+  const int tid = local_tid();
+  in.j = st->j;
+  in.fjb = st->fjb;
+  in.k1 = st->k1;
+  // e walks rows fastest: coalesced column-major reads
+#pragma unroll
+  for (int q = 0; q < KMAX * KMAX / NT1; q++) {
+    const int e = tid + NT1 * q, r = e & 63, c = e >> 6;
+    const bool in_blk = r < in.k1 && c < in.k1;
+    in.gr[q] = R1[r + c * KMAX];
+    in.pri[q] = R1inv[r + c * KMAX];
+    in.ptop[q] = in_blk ? A[(size_t)(in.j + c) * lda + in.j + r] : 0.0;
+  }
+}
+
+// The reconstruction of the DM loop's Gram-based panel: G2 from memory, the
+// rest from rin; writes st->fjb_cmp and st->c0, or the fallback flag.
+__device__ __forceinline__ void recon_body(const double* __restrict__ G2, const ReconIn& rin,
+                                           double* A, int lda, double tau_dm, double* tau,
+                                           DevState* st, double* M_out) {
   __shared__ double SR[KMAX][LDP];  // R1, then R, then L^T, then U^-1 (rows scaled by D)
   __shared__ double SA[KMAX][LDP];  // R^-1
   __shared__ double SW[KMAX][LDP];  // E, R2^-1, then Q11 and its LU, then L^-T
@@ -971,30 +1000,23 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   __shared__ double Y[4][BLK][BLK + 1];
   __shared__ double rinv[KMAX], dsg[KMAX], tl[KMAX], scv[KMAX];
   __shared__ int s_done, s_why;
-  if (!step_runs(st, RUN_FAST)) return;
-  const int j = st->j, fjb = st->fjb;
+  const int j = rin.j, fjb = rin.fjb, k1 = rin.k1;
   const int tid = threadIdx.x;
-  const int k1 = st->k1;
-  // every global read of the reconstruction in one round trip: G2, R1, and,
-  // kept in registers until their stage, R1^-1 and the panel's top block
-  // (e walks rows fastest: coalesced column-major reads)
-  double pri[KMAX * KMAX / NT1], ptop[KMAX * KMAX / NT1];
+  // G2 from the launch's reduce (the one global read left); R1 from the
+  // registers read before the ticket
   {
-    double gx[KMAX * KMAX / NT1], gr[KMAX * KMAX / NT1];
+    double gx[KMAX * KMAX / NT1];
 #pragma unroll
     for (int q = 0; q < KMAX * KMAX / NT1; q++) {
       const int e = tid + NT1 * q, r = e & 63, c = e >> 6;
       const bool in = r < k1 && c < k1;
-      gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);  // from the launch's reduce
-      gr[q] = R1[r + c * KMAX];
-      pri[q] = R1inv[r + c * KMAX];
-      ptop[q] = in ? A[(size_t)(j + c) * lda + j + r] : 0.0;
+      gx[q] = in ? ld_dev(G2 + r + c * KMAX) : (r == c ? 1.0 : 0.0);
     }
 #pragma unroll
     for (int q = 0; q < KMAX * KMAX / NT1; q++) {
       const int e = tid + NT1 * q, r = e & 63, c = e >> 6;
       SX[r][c] = gx[q];
-      SR[r][c] = gr[q];
+      SR[r][c] = rin.gr[q];
     }
   }
   lds_barrier();
@@ -1007,7 +1029,8 @@ __device__ void recon_body(const double* __restrict__ G2, const double* __restri
   };
   const int k = recon_core(
       SR, SA, SW, SX, Y, rinv, dsg, tl, scv, &s_done, &s_why,
-      [&](double(*S)[LDP]) { from_regs(pri, S); }, [&](double(*S)[LDP]) { from_regs(ptop, S); }, A,
+      [&](double(*S)[LDP]) { from_regs(rin.pri, S); },
+      [&](double(*S)[LDP]) { from_regs(rin.ptop, S); }, A,
       lda, j, fjb, k1, tau_dm, tau + j, M_out);
   if (k < 0) {
     if (tid == 0) {
@@ -1038,6 +1061,8 @@ __global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__
   __shared__ double part[4][64];
   __shared__ int s_last;
   if (!step_runs(st, RUN_FAST)) return;
+  ReconIn rin;
+  recon_inputs(rin, st, R1, R1inv, A, lda);
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
   const int q = threadIdx.x >> 6;
   double sacc = 0.0;
@@ -1061,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_reduce_recon(const double* __restrict__
   if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  recon_body(G2, R1, R1inv, A, lda, tau_dm, tau, st, M_out);
+  recon_body(G2, rin, A, lda, tau_dm, tau, st, M_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1151,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_reduce_t(const double* __restrict__ par
   __shared__ double rinv[KMAX];
   __shared__ int s_last;
   if (!step_runs(st, RUN_FAST)) return;
-  const int j = st->j;
+  const int j = st->j, k = st->fjb_cmp;
   const int nslab = (m - j + PR_ROWS - 1) / PR_ROWS;
   const int e = blockIdx.x * 64 + (threadIdx.x & 63);
   const int q = threadIdx.x >> 6;
@@ -1176,7 +1201,6 @@ __global__ __launch_bounds__(256) void k_reduce_t(const double* __restrict__ par
   if (threadIdx.x == 0) s_last = last_to_arrive(ticket, gridDim.x) ? 1 : 0;
   __syncthreads();
   if (!s_last) return;
-  const int k = st->fjb_cmp;
   const int tid = threadIdx.x;
   {
     // every load in flight before the LDS stores (one round trip)
