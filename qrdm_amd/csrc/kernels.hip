@@ -812,9 +812,20 @@ __global__ __launch_bounds__(256) void k_gram_cand(double* A, int lda, int m, do
     // X's layout.  The updated rows are written back.
     const double* Vp = v_prev(bufs, st);
     const double* Wq = w_prev(bufs, st);
-    for (int e = tid; e < KMAX * KMAX; e += 256) {
-      const int i = e & 63, c = e >> 6;  // a candidate's 64 entries are contiguous
-      Ws[i][c] = (c < nc) ? Wq[(size_t)(j + st->cand_idx[c]) * KMAX + i] : 0.0;
+    {
+      // wave wv gathers candidates wv, wv + 4, ... (a candidate's 64 entries
+      // are contiguous): the indices from one load per lane, every W load
+      // in flight before the LDS stores
+      const int cid = st->cand_idx[lane];
+      double wc[KMAX / 4];
+#pragma unroll
+      for (int q = 0; q < KMAX / 4; q++) {
+        const int c = wv + 4 * q;
+        const int ci = __builtin_amdgcn_readlane(cid, c);
+        wc[q] = (c < nc) ? Wq[(size_t)(j + ci) * KMAX + lane] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < KMAX / 4; q++) Ws[lane][wv + 4 * q] = wc[q];
     }
     double va[KMAX / 4];
 #pragma unroll
