@@ -75,11 +75,12 @@ __host__ __device__ __forceinline__ FuGeom fused_geom(int r_lo, int m, int c0, i
   return g;
 }
 
-// Workgroups of a fused pass, dealt to the eight XCDs in contiguous runs of
-// (slab, column group) so that one XCD's workgroups share few row slabs of V
-// in its L2.
+// Workgroups of a fused pass per XCD: column group g goes to XCD g % 8 with
+// all its row slabs, so that the step's end (one workgroup per 64-column
+// tile, dispatched round-robin to the XCDs: tile g on XCD g % 8) reads the
+// slab partials of its columns from its own L2.
 __host__ __device__ __forceinline__ int fused_per_xcd(int ns, int cgroups) {
-  return (ns * cgroups + 7) / 8;
+  return ((cgroups + 7) / 8) * ns;
 }
 
 // One pass.  APPLY: C -= Vp Wprev (Vp, Wprev of the previous step) before
@@ -113,9 +114,8 @@ __global__ __launch_bounds__(64 * FU_NWG, FuShape<U>::OCC) void k_fused(
   const int per = fused_per_xcd(ns, G.cgroups);
   const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
   if (k >= per) return;
-  const int L = xcd * per + k;
-  if (L >= ns * G.cgroups) return;
-  const int slab = L / G.cgroups, cgrp = L % G.cgroups;
+  const int slab = k % ns, cgrp = xcd + 8 * (k / ns);
+  if (cgrp >= G.cgroups) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, li = lane & 15;
   const int ncol = n - c0;
@@ -289,12 +289,13 @@ static int fused_wgs(int u) { return 256 * (u == 1 ? FuShape<1>::OCC : FuShape<2
 
 int fused_max_slabs() { return FU_MAX_SPLIT; }
 
-// Row slabs of a fused pass: one round of workgroups where the column groups
-// allow it.
+// Row slabs of a fused pass: one round of workgroups on every XCD (its
+// column groups times the slabs within an eighth of the chip's slots) where
+// the column groups allow it.
 int fused_slabs(int r_lo, int m, int c0, int n) {
   const FuGeom g = fused_geom(r_lo, m, c0, n, 1, fused_gc(g_fused_u));
   if (g.cgroups <= 0 || g.nchunk <= 0) return 1;
-  int ns = fused_wgs(g_fused_u) / g.cgroups;
+  int ns = (fused_wgs(g_fused_u) / 8) / ((g.cgroups + 7) / 8);
   if (ns > FU_MAX_SPLIT) ns = FU_MAX_SPLIT;
   if (ns > g.nchunk) ns = g.nchunk;
   return ns < 1 ? 1 : ns;
